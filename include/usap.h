@@ -1,0 +1,216 @@
+/*
+ * usap.h -- C-ABI of the MI355X stochastic block partitioner engine.
+ *
+ * Plain C types only (pointers and sizes); no C++ or torch types cross this
+ * boundary.  All functions return USAP_OK (0) or a negative USAP_E* code and
+ * never throw or exit.  One engine handle owns one HIP stream and all device
+ * memory of one graph; a handle is used from one host thread at a time, and
+ * several handles may run concurrently.
+ *
+ * The engine is the drop-in for the hot path of sgp::Graph_P<W>::partition()
+ * in chih-chun-chang/uSAP include/graph_parallel.hpp (the reference has no
+ * FFI; its path sits behind private member functions called from Taskflow
+ * tasks).  Each entry point names the reference function it replaces.
+ *
+ * Arithmetic shared with the CPU oracle:
+ *   lambda(x) = llrint(log(x) * 2^48), T(m,x,y) = m*(lambda(m)-lambda(x)-lambda(y))
+ *   summed exactly in int128 (scale 2^-48); a dS is reported as a double and,
+ *   through the *_fx entry points, as its exact int128 (hi, lo) words.
+ * Random stream: Philox4x64-10, key {seed, 0}, counter {item, proposal,
+ *   stream, tag}; slots 0 neighbour, 1 accept-uniform, 2 uniform block,
+ *   3 multinomial; an index in [0, W) is (x * W) >> 64.
+ */
+#ifndef USAP_H
+#define USAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define USAP_OK 0
+#define USAP_EINVAL (-1)       /* bad argument */
+#define USAP_ENOMEM (-2)       /* host or device allocation failed */
+#define USAP_EHIP (-3)         /* HIP runtime error (no device, launch failure) */
+#define USAP_ESTATE (-4)       /* call not valid in this state (e.g. golden-section bracket invalid) */
+#define USAP_EINFEASIBLE (-5)  /* carry-out ran out of candidate merges */
+#define USAP_EIO (-6)          /* file could not be read or parsed */
+
+#define USAP_INIT_SCC 0        /* graph_parallel.hpp:514-556 (reference default) */
+#define USAP_INIT_IDENTITY 1   /* B starts at N */
+#define USAP_DS_REF 0          /* nodal dS of graph_parallel.hpp:1066-1215 */
+#define USAP_DS_EXACT 1        /* nodal dS with the corner in-edge updates of graph.cu:1414-1426 */
+
+/* A directed graph as CSR (out-edges) + CSC (in-edges), both in the
+ * reference's neighbour order (file order, graph_parallel.hpp:290-294).
+ * Ids are 0-based. */
+typedef struct usap_graph {
+  int32_t N;
+  int64_t E;
+  const int64_t* out_ptr; /* [N+1] */
+  const int32_t* out_idx; /* [E] */
+  const int64_t* in_ptr;  /* [N+1] */
+  const int32_t* in_idx;  /* [E] */
+  const int32_t* out_w;   /* [E], NULL = unweighted (all 1) */
+  const int32_t* in_w;    /* [E], NULL = unweighted */
+} usap_graph;
+
+/* Parameters of Graph_P (graph_parallel.hpp:41-52) plus engine options. */
+typedef struct usap_params {
+  int32_t K;                       /* num_agg_proposals_per_block = 10 */
+  double reduction_rate;           /* num_block_reduction_rate = 0.5 */
+  int32_t max_nodal_itr;           /* max_num_nodal_itr = 100 */
+  int32_t num_batch_nodal_update;  /* 4: nodal batch = N / 4 */
+  int32_t nodal_batch;             /* > 0: fixed nodal batch size (overrides the above) */
+  double thr1;                     /* delta_entropy_threshold1 = 5e-4 */
+  double thr2;                     /* delta_entropy_threshold2 = 1e-4 */
+  int32_t avg_window;              /* delta_entropy_moving_avg_window = 3 */
+  int32_t dfs_depth;               /* dfs_depth = 20 */
+  int32_t init;                    /* USAP_INIT_SCC | USAP_INIT_IDENTITY */
+  int32_t ds_mode;                 /* USAP_DS_REF | USAP_DS_EXACT */
+  uint64_t seed;                   /* random-stream key */
+  int32_t device;                  /* HIP device ordinal */
+  int32_t verbose;                 /* print per-phase progress to stderr */
+} usap_params;
+
+/* Partition statistics (usap_partition).  Times are host wall-clock ms. */
+typedef struct usap_stats {
+  double S;                 /* final MDL */
+  int32_t B;                /* final block count */
+  int32_t phases;           /* merge phases run */
+  int64_t sweeps;           /* nodal sweeps */
+  int64_t batches;          /* nodal batches */
+  int64_t nodal_proposals;  /* nodal proposals evaluated (node-moves) */
+  int64_t accepted_moves;   /* accepted nodal moves */
+  int64_t merge_proposals;  /* block-merge proposals evaluated */
+  double ms_total;          /* partition() window, graph upload excluded */
+  double ms_merge;          /* merge proposals */
+  double ms_carry;          /* carry-out + relabel + M rebuild */
+  double ms_nodal;          /* nodal batches (propose + apply) */
+  double ms_entropy;        /* MDL evaluations */
+  double ms_golden;         /* snapshot / restore */
+  int32_t rebuilds;         /* full M rebuilds */
+  int32_t error;            /* 0 or USAP_E* */
+} usap_stats;
+
+typedef struct usap_engine usap_engine;
+
+/* Defaults of graph_parallel.hpp:41-52 (seed 0, device 0, SCC init, ref dS). */
+void usap_default_params(usap_params* p);
+
+/* Create an engine: uploads the graph, builds the lambda table, allocates
+ * device state for up to N blocks.  ≙ Graph_P ctor (graph_parallel.hpp:61-78)
+ * minus the file load (see usap_load_tsv). */
+int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out);
+int usap_destroy(usap_engine* e);
+const char* usap_last_error(const usap_engine* e);
+
+/* Set labels (all in [0,B)) and rebuild M and the block degrees.
+ * ≙ labels assignment + _initialize_edge_counts (graph_parallel.hpp:703-759). */
+int usap_set_labels(usap_engine* e, const int32_t* labels, int32_t B);
+int usap_get_labels(usap_engine* e, int32_t* out);
+int usap_num_blocks(usap_engine* e, int32_t* B);
+/* Block degrees d_out/d_in [B]. ≙ Partition::d (graph_parallel.hpp:85-99). */
+int usap_get_degrees(usap_engine* e, int32_t* d_out, int32_t* d_in);
+/* M as COO triplets sorted by (row, col), zero entries omitted; *nnz gets the
+ * count (call with cap 0 to query).  ≙ Partition::M / Mrow (graph_parallel.hpp:101-110). */
+int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, int32_t* vals,
+                   int64_t* nnz);
+
+/* MDL S = model_S + data_S.  ≙ _compute_overall_entropy (graph_parallel.hpp:1281-1319).
+ * data_fx_hi/lo (nullable): exact data_S as int128 words. */
+int usap_entropy(usap_engine* e, double* S, int64_t* data_fx_hi, uint64_t* data_fx_lo);
+
+/* K merge proposals per block and the best per block.
+ * ≙ block_merge task (graph_parallel.hpp:349-368) over
+ *   _propose_new_partition_block (graph_parallel.hpp:761-988).
+ * best_s/best_dS (nullable, [B]) receive the result. */
+int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best_s,
+                       double* best_dS);
+/* Exact dS words of the last usap_merge_propose ([B] each). */
+int usap_merge_result_fx(usap_engine* e, int32_t* best_s, int64_t* dS_hi, uint64_t* dS_lo);
+
+/* Carry out B_to_merge merges of the last proposal round, relabel, compact,
+ * rebuild M.  ≙ _carry_out_best_merges (graph_parallel.hpp:1225-1279) +
+ * _initialize_edge_counts.  USAP_EINFEASIBLE if the candidates run out
+ * (undefined behaviour in the reference); the state is then unchanged. */
+int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new);
+
+/* One nodal batch over nodes [begin,end): proposals against the frozen M,
+ * greedy accept (dS < 0), then the accepted moves applied to labels, M and
+ * degrees.  ≙ nodal_move + update tasks (graph_parallel.hpp:406-458) over
+ * _propose_new_partition_nodal (graph_parallel.hpp:990-1222).
+ * dS_sum = sum of accepted dS (may be -inf in ref mode). */
+int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream,
+                     double* dS_sum, int32_t* n_moves);
+/* Exact sum of the last batch: int128 words and inf flag (0 or -1). */
+int usap_nodal_sum_fx(usap_engine* e, int64_t* hi, uint64_t* lo, int32_t* inf);
+/* Per-node results of the last batch ([end-begin] each, nullable):
+ * proposed block, dS words, inf flag (0 finite, -1 -inf), accepted. */
+int usap_nodal_detail(usap_engine* e, int32_t* s, int64_t* dS_hi, uint64_t* dS_lo, int32_t* inf,
+                      int32_t* accept);
+
+/* Snapshot slots (0..2) hold labels, B and S.  ≙ OldData copies
+ * (graph_parallel.hpp:111-124, :1327-1362). */
+int usap_snapshot(usap_engine* e, int32_t slot, double S);
+int usap_restore(usap_engine* e, int32_t slot);
+
+/* Full partition(): init (params.init), then merge phases, nodal sweeps and
+ * the golden-section search over B, run by the host C++ driver over the
+ * entry points above.  ≙ Graph_P::partition (graph_parallel.hpp:322-512) +
+ * _prepare_for_partition_next (graph_parallel.hpp:1322-1365).
+ * labels_out (nullable, [N]); stats (nullable). */
+int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats);
+
+/* Golden-section bracket update on (B, S, id) triples, exposed for tests.
+ * ≙ _prepare_for_partition_next (graph_parallel.hpp:1322-1365).
+ * slots_* [3] = large, med, small (B = 0, S = +inf, id = -1 when empty).
+ * Returns 1 if the optimal B is found, 0 to continue, USAP_ESTATE if the
+ * bracket is invalid. */
+int usap_golden_update(int64_t* slot_B, double* slot_S, int64_t* slot_id, int64_t cur_B,
+                       double cur_S, int64_t cur_id, double rate, int64_t* next_B,
+                       double* next_S, int64_t* next_id, int64_t* B_to_merge);
+
+/* ---- host utilities (no device needed) ---------------------------------- */
+
+/* SCC initial partition.  ≙ _scc + _dfs_itr (graph_parallel.hpp:514-603). */
+int usap_scc_init(const usap_graph* g, int32_t dfs_depth, int32_t* labels, int32_t* B);
+
+/* Model entropy term.  ≙ graph_parallel.hpp:1314-1316 in double. */
+double usap_model_S(int64_t N, int64_t E, int64_t B);
+
+/* lambda table [maxv+1] (host copy of what the engine uploads). */
+int usap_lambda_table(int64_t maxv, int64_t* out);
+
+/* Philox4x64-10 on the host: out[4] = philox(ctr[4], key[2]). */
+void usap_philox4x64(const uint64_t* ctr, const uint64_t* key, uint64_t* out);
+
+/* Build CSR/CSC from an edge list in file order (stable by source / target).
+ * src/dst 0-based; w nullable.  Output arrays sized by the caller:
+ * out_ptr/in_ptr [N+1], out_idx/in_idx [E], out_w/in_w [E] (nullable). */
+int usap_build_csr(int32_t N, int64_t E, const int32_t* src, const int32_t* dst,
+                   const int32_t* w, int64_t* out_ptr, int32_t* out_idx, int32_t* out_w,
+                   int64_t* in_ptr, int32_t* in_idx, int32_t* in_w);
+
+/* Read a GraphChallenge TSV ("from\tto\tweight", 1-based).  ≙ load_graph_from_tsv
+ * (graph_parallel.hpp:251-317), except N = max id over both columns.  Call
+ * with NULL arrays to query N and E, then again with arrays of that size. */
+int usap_load_tsv(const char* path, int32_t* N, int64_t* E, int32_t* src, int32_t* dst,
+                  int32_t* w);
+
+/* Seeded directed degree-corrected SBM: K planted blocks with sizes
+ * min_block + Dirichlet(alpha) share of the rest, out-degree power law
+ * d^-gamma on [dmin, dmax], a fraction p_in of edges inside the block and the
+ * rest to a uniformly chosen other block, no self-loops or duplicates.
+ * Edges come out grouped by source ascending.  Call with NULL src/dst/truth
+ * to query E (deterministic), then again with arrays. */
+int usap_generate_dcsbm(int32_t N, int32_t K, double alpha, int32_t min_block, int32_t dmin,
+                        int32_t dmax, double gamma, double p_in, uint64_t seed, int64_t* E,
+                        int32_t* src, int32_t* dst, int32_t* truth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,152 @@
+"""ctypes bindings of the C-ABI in include/usap.h (usap_amd/lib/libusap.so).
+
+The library is the product: if it is missing, every entry point raises.  There
+is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libusap.so")
+
+USAP_OK = 0
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "EINFEASIBLE", -6: "EIO"}
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+class UsapError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"usap error {code} ({ERRORS.get(code, '?')}): {msg}")
+
+
+class UsapGraph(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32),
+        ("E", C.c_int64),
+        ("out_ptr", _vp),
+        ("out_idx", _vp),
+        ("in_ptr", _vp),
+        ("in_idx", _vp),
+        ("out_w", _vp),
+        ("in_w", _vp),
+    ]
+
+
+class UsapParams(C.Structure):
+    _fields_ = [
+        ("K", C.c_int32),
+        ("reduction_rate", C.c_double),
+        ("max_nodal_itr", C.c_int32),
+        ("num_batch_nodal_update", C.c_int32),
+        ("nodal_batch", C.c_int32),
+        ("thr1", C.c_double),
+        ("thr2", C.c_double),
+        ("avg_window", C.c_int32),
+        ("dfs_depth", C.c_int32),
+        ("init", C.c_int32),
+        ("ds_mode", C.c_int32),
+        ("seed", C.c_uint64),
+        ("device", C.c_int32),
+        ("verbose", C.c_int32),
+    ]
+
+
+class UsapStats(C.Structure):
+    _fields_ = [
+        ("S", C.c_double),
+        ("B", C.c_int32),
+        ("phases", C.c_int32),
+        ("sweeps", C.c_int64),
+        ("batches", C.c_int64),
+        ("nodal_proposals", C.c_int64),
+        ("accepted_moves", C.c_int64),
+        ("merge_proposals", C.c_int64),
+        ("ms_total", C.c_double),
+        ("ms_merge", C.c_double),
+        ("ms_carry", C.c_double),
+        ("ms_nodal", C.c_double),
+        ("ms_entropy", C.c_double),
+        ("ms_golden", C.c_double),
+        ("rebuilds", C.c_int32),
+        ("error", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every exported entry point of include/usap.h
+SIGNATURES = [
+    ("usap_default_params", None, [C.POINTER(UsapParams)]),
+    ("usap_create", C.c_int, [C.POINTER(UsapGraph), C.POINTER(UsapParams), C.POINTER(_vp)]),
+    ("usap_destroy", C.c_int, [_vp]),
+    ("usap_last_error", C.c_char_p, [_vp]),
+    ("usap_set_labels", C.c_int, [_vp, _i32p, C.c_int32]),
+    ("usap_get_labels", C.c_int, [_vp, _i32p]),
+    ("usap_num_blocks", C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    ("usap_get_degrees", C.c_int, [_vp, _i32p, _i32p]),
+    ("usap_get_M_coo", C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(C.c_int64)]),
+    ("usap_entropy", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                               C.POINTER(C.c_uint64)]),
+    ("usap_merge_propose", C.c_int, [_vp, C.c_int32, C.c_uint64, _vp, _vp]),
+    ("usap_merge_result_fx", C.c_int, [_vp, _i32p, _i64p, _u64p]),
+    ("usap_merge_apply", C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),
+    ("usap_nodal_batch", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_uint64,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("usap_nodal_sum_fx", C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_int32)]),
+    ("usap_nodal_detail", C.c_int, [_vp, _i32p, _i64p, _u64p, _i32p, _i32p]),
+    ("usap_snapshot", C.c_int, [_vp, C.c_int32, C.c_double]),
+    ("usap_restore", C.c_int, [_vp, C.c_int32]),
+    ("usap_partition", C.c_int, [_vp, _vp, C.POINTER(UsapStats)]),
+    ("usap_golden_update", C.c_int, [_i64p, _f64p, _i64p, C.c_int64, C.c_double, C.c_int64,
+                                     C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("usap_scc_init", C.c_int, [C.POINTER(UsapGraph), C.c_int32, _i32p, C.POINTER(C.c_int32)]),
+    ("usap_model_S", C.c_double, [C.c_int64, C.c_int64, C.c_int64]),
+    ("usap_lambda_table", C.c_int, [C.c_int64, _i64p]),
+    ("usap_philox4x64", None, [_u64p, _u64p, _u64p]),
+    ("usap_build_csr", C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp]),
+    ("usap_load_tsv", C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64), _vp,
+                                _vp, _vp]),
+    ("usap_generate_dcsbm", C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32,
+                                      C.c_int32, C.c_double, C.c_double, C.c_uint64,
+                                      C.POINTER(C.c_int64), _vp, _vp, _vp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libusap.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build it with `python -m usap_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, eng=None):
+    if rc != USAP_OK:
+        msg = lib().usap_last_error(eng).decode() if eng else ""
+        raise UsapError(rc, msg)
+    return rc
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data

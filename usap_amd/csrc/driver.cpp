@@ -1,0 +1,257 @@
+// driver.cpp -- host C++ restatement of Graph_P::partition()
+// (graph_parallel.hpp:322-512) and its golden-section search over B
+// (graph_parallel.hpp:1322-1365), running the hot path through the engine's
+// C-ABI (usap_merge_propose / usap_merge_apply / usap_nodal_batch /
+// usap_entropy / usap_snapshot / usap_restore).
+//
+// Random streams: merge phase p draws with stream p, the j-th nodal batch of
+// the whole run with stream j.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "engine_internal.hpp"
+#include "usap.h"
+#include "usap_common.hpp"
+
+namespace {
+
+using usap::i128;
+using usap::u128;
+
+struct Snap {
+  int64_t B = 0;
+  double S = INFINITY;
+  int64_t id = -1;
+};
+
+double fx_to_double(i128 v) {
+  const bool neg = v < 0;
+  const u128 m = neg ? (u128)0 - (u128)v : (u128)v;
+  const uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
+  const double d = std::ldexp((double)hi, 64 - usap::kLambdaFrac) +
+                   std::ldexp((double)lo, -usap::kLambdaFrac);
+  return neg ? -d : d;
+}
+
+// _prepare_for_partition_next, graph_parallel.hpp:1322-1365.  Returns 1 when
+// the optimal B is found, 0 to continue, USAP_ESTATE when the bracket is
+// invalid (large never set while small is: the reference's size_t
+// arithmetic is undefined there).
+int golden(Snap& large, Snap& med, Snap& small, const Snap& cur, double rate, Snap& next,
+           int64_t& B_to_merge) {
+  if (cur.S <= med.S) {
+    if (med.B > cur.B)
+      large = med;
+    else
+      small = med;
+    med = cur;
+  } else {
+    if (med.B > cur.B)
+      small = cur;
+    else
+      large = cur;
+  }
+  B_to_merge = 0;
+  if (std::isinf(small.S)) {
+    next = med;
+    B_to_merge = (int64_t)std::floor((double)med.B * rate);
+    return B_to_merge == 0 ? 1 : 0;
+  }
+  if (large.id < 0) return USAP_ESTATE;
+  if (large.B - small.B == 2) {
+    next = med;
+    return 1;
+  }
+  if ((large.B - med.B) >= (med.B - small.B)) {
+    const int64_t nb = med.B + (int64_t)std::round((double)(large.B - med.B) * 0.618);
+    next = large;
+    B_to_merge = large.B - nb;
+  } else {
+    const int64_t nb = small.B + (int64_t)std::round((double)(med.B - small.B) * 0.618);
+    next = med;
+    B_to_merge = med.B - nb;
+  }
+  return 0;
+}
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0) {
+  return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int usap_golden_update(int64_t* slot_B, double* slot_S, int64_t* slot_id, int64_t cur_B,
+                       double cur_S, int64_t cur_id, double rate, int64_t* next_B,
+                       double* next_S, int64_t* next_id, int64_t* B_to_merge) {
+  if (!slot_B || !slot_S || !slot_id || !next_B || !next_S || !next_id || !B_to_merge)
+    return USAP_EINVAL;
+  Snap s[3];
+  for (int i = 0; i < 3; ++i) s[i] = Snap{slot_B[i], slot_S[i], slot_id[i]};
+  Snap cur{cur_B, cur_S, cur_id}, next;
+  int64_t btm = 0;
+  const int rc = golden(s[0], s[1], s[2], cur, rate, next, btm);
+  for (int i = 0; i < 3; ++i) {
+    slot_B[i] = s[i].B;
+    slot_S[i] = s[i].S;
+    slot_id[i] = s[i].id;
+  }
+  *next_B = next.B;
+  *next_S = next.S;
+  *next_id = next.id;
+  *B_to_merge = btm;
+  return rc;
+}
+
+int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
+  if (!e) return USAP_EINVAL;
+  const usap_graph* g = usap::engine_host_graph(e);
+  const usap_params& p = *usap::engine_params(e);
+  const int32_t N = g->N;
+  usap_stats st{};
+  const int rebuilds0 = usap::engine_rebuilds(e);
+  const auto t_all = clk::now();
+  int rc = USAP_OK;
+
+  // init_merge (graph_parallel.hpp:338-342)
+  std::vector<int32_t> lab((size_t)N);
+  int32_t B = 0;
+  if (p.init == USAP_INIT_IDENTITY) {
+    std::iota(lab.begin(), lab.end(), 0);
+    B = N;
+  } else {
+    if ((rc = usap_scc_init(g, p.dfs_depth, lab.data(), &B))) return rc;
+  }
+  if ((rc = usap_set_labels(e, lab.data(), B))) return rc;
+  int64_t B_to_merge = (int64_t)std::floor((double)B * p.reduction_rate);
+
+  int32_t batch = p.nodal_batch > 0 ? p.nodal_batch : N / p.num_batch_nodal_update;
+  if (batch < 1) batch = 1;
+  const int32_t W = p.avg_window;
+  std::vector<double> itr_dS((size_t)p.max_nodal_itr + 1, 0.0);
+  Snap large, med, small;
+  // snapshot slot of each partition id held by the golden-section bracket
+  std::vector<int> slot_of;
+  bool slot_used[4] = {false, false, false, false};
+  uint64_t merge_stream = 0, nodal_stream = 0;
+  double S = 0;
+
+  for (int64_t phase = 0;; ++phase) {
+    // prepare_merge + block_merge + perform_merge + prepare_move
+    auto t0 = clk::now();
+    if ((rc = usap_merge_propose(e, p.K, merge_stream++, nullptr, nullptr))) break;
+    st.merge_proposals += (int64_t)B * p.K;
+    st.ms_merge += ms_since(t0);
+    t0 = clk::now();
+    if ((rc = usap_merge_apply(e, (int32_t)B_to_merge, &B))) break;
+    st.ms_carry += ms_since(t0);
+    st.phases++;
+    std::fill(itr_dS.begin(), itr_dS.end(), 0.0);
+    for (int32_t itr = 0; itr < p.max_nodal_itr; ++itr) {
+      // prepare_batch
+      t0 = clk::now();
+      if ((rc = usap_entropy(e, &S, nullptr, nullptr))) break;
+      st.ms_entropy += ms_since(t0);
+      double mean = 0;
+      if (itr >= W - 1) {
+        for (int32_t i = itr - W + 1; i < itr; ++i) mean += itr_dS[i];
+        mean /= (double)(W - 1);
+      }
+      const bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
+      i128 cum = 0;
+      bool cum_inf = false, converged = false;
+      st.sweeps++;
+      t0 = clk::now();
+      for (int32_t beg = 0; beg < N; beg += batch) {
+        const int32_t end = beg + batch > N ? N : beg + batch;
+        int32_t moves = 0;
+        if ((rc = usap_nodal_batch(e, beg, end, nodal_stream++, nullptr, &moves))) break;
+        int64_t hi;
+        uint64_t lo;
+        int32_t inf;
+        usap_nodal_sum_fx(e, &hi, &lo, &inf);
+        st.batches++;
+        st.nodal_proposals += end - beg;
+        st.accepted_moves += moves;
+        if (inf)
+          cum_inf = true;
+        else
+          cum += (i128)(((u128)(uint64_t)hi << 64) | (u128)lo);
+        const double dS_sum = cum_inf ? -INFINITY : fx_to_double(cum);
+        itr_dS[itr] = dS_sum;
+        S += dS_sum;
+        if (itr >= W - 1) {
+          const double thr = isf ? p.thr2 : p.thr1;
+          if (-mean < thr * S) {
+            converged = true;
+            break;
+          }
+        }
+      }
+      st.ms_nodal += ms_since(t0);
+      if (rc || converged) break;
+    }
+    if (rc) break;
+    // prepare_for_next
+    t0 = clk::now();
+    if ((rc = usap_entropy(e, &S, nullptr, nullptr))) break;
+    st.ms_entropy += ms_since(t0);
+    t0 = clk::now();
+    int slot = -1;
+    for (int i = 0; i < 4; ++i)
+      if (!slot_used[i]) {
+        slot = i;
+        break;
+      }
+    if (slot < 0) {
+      rc = USAP_ESTATE;
+      break;
+    }
+    if ((rc = usap_snapshot(e, slot, S))) break;
+    slot_used[slot] = true;
+    slot_of.push_back(slot);
+    Snap cur{B, S, phase}, next;
+    const int gr = golden(large, med, small, cur, p.reduction_rate, next, B_to_merge);
+    if (gr < 0) {
+      rc = gr;
+      usap::engine_set_error(e, "golden-section bracket invalid (large never set)");
+      break;
+    }
+    if (next.id != phase) {
+      if ((rc = usap_restore(e, slot_of[next.id]))) break;
+      B = (int32_t)next.B;
+    }
+    for (int64_t id = 0; id <= phase; ++id) {
+      const int sl = slot_of[id];
+      if (sl < 0) continue;
+      if (id == large.id || id == med.id || id == small.id || id == next.id) continue;
+      slot_used[sl] = false;
+      slot_of[id] = -1;
+    }
+    st.ms_golden += ms_since(t0);
+    if (p.verbose)
+      std::fprintf(stderr, "usap: phase %lld B=%d S=%.6f bracket=[%lld,%lld,%lld] next B=%lld\n",
+                   (long long)phase, B, S, (long long)large.B, (long long)med.B,
+                   (long long)small.B, (long long)(next.B - B_to_merge));
+    if (gr == 1) {
+      S = next.S;
+      break;
+    }
+  }
+  st.ms_total = ms_since(t_all);
+  st.S = S;
+  st.B = B;
+  st.rebuilds = usap::engine_rebuilds(e) - rebuilds0;
+  st.error = rc;
+  if (stats) *stats = st;
+  if (rc) return rc;
+  if (labels_out) rc = usap_get_labels(e, labels_out);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,1645 @@
+// engine.hip -- MI355X (gfx950) engine for uSAP's partition hot path.
+//
+// Device state per graph: CSR/CSC (int64 offsets, int32 ids), labels int32[N],
+// block degrees int32[B], and the interblock edge-count matrix M kept twice
+// (rows of M and rows of M^T = columns of M) in one slab pool per orientation:
+//   row a occupies slots [off[a], off[a]+cap[a]);
+//   cap[a] == B  -> direct-mapped row (slot = column), used when a hashed row
+//                   would need >= B slots (dense regime, small B);
+//   cap[a] <  B  -> open-addressing hash (power of two, linear probing),
+//                   keys int32 (-1 empty), counts int32.
+// Keys are never deleted inside a sweep (a count may drop to 0); a row whose
+// key count passes 3/4 of its slab raises an overflow flag and the host
+// rebuilds M from the labels after that batch.
+//
+// Kernels (one wavefront of 64 lanes per proposal item):
+//   k_degrees / k_caps / k_fill / k_insert_edges : M build from labels
+//        (graph_parallel.hpp:703-759)
+//   k_merge   : K merge proposals per block + best per block
+//        (graph_parallel.hpp:349-368, :761-988), one workgroup per block,
+//        one wavefront per proposal
+//   k_nodal   : one nodal proposal per node against the frozen M
+//        (graph_parallel.hpp:990-1222), one wavefront per node
+//   k_apply / k_labels : accepted moves -> M, degrees (int32 atomics) and
+//        labels; equals the reference's full rebuild (graph_parallel.hpp:438-443)
+//   k_entropy : data_S over all non-zeros (graph_parallel.hpp:1281-1319)
+//
+// dS uses the factorised form of the reference's term sums: terms whose count
+// and degrees do not change cancel exactly in integer arithmetic, so a nodal
+// proposal costs O(distinct neighbour blocks) lookups and a merge proposal
+// O(min row length) lookups, instead of the reference's O(B) scans.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <rocprim/rocprim.hpp>
+#include <string>
+#include <vector>
+
+#include "engine_internal.hpp"
+#include "usap.h"
+#include "usap_common.hpp"
+
+using usap::i128;
+using usap::u128;
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kNodalWaves = 4;           // wavefronts per workgroup in k_nodal / k_apply
+constexpr int kTab = 512;                // per-wavefront LDS table slots
+constexpr int kTabMaxDeg = kTab * 3 / 4; // larger nodes use a global-memory table
+constexpr int kMaxMergeWaves = 16;
+constexpr int kEntropyBlocks = 1024;
+constexpr int kNumSnap = 4;
+
+enum : int32_t {
+  kFlagLoad = 1,      // a hashed row passed 3/4 load
+  kFlagFull = 2,      // an insertion found no free slot (increment lost)
+  kFlagMissing = 4,   // a decrement did not find its key (bug)
+  kFlagNegative = 8,  // a negative count reached an entropy term (bug)
+};
+
+struct Status {
+  unsigned long long sum_lo;
+  unsigned long long sum_hi;
+  int32_t ninf;
+  int32_t moves;
+  int32_t flags;
+  int32_t pad;
+};
+
+struct GraphView {
+  int32_t N;
+  const int64_t* out_ptr;
+  const int32_t* out_idx;
+  const int32_t* out_w;
+  const int64_t* in_ptr;
+  const int32_t* in_idx;
+  const int32_t* in_w;
+};
+
+// One orientation of M, as seen by kernels.
+struct MView {
+  const int64_t* off;
+  const int32_t* cap;
+  int32_t* nkeys;
+  int32_t* key;
+  int32_t* val;
+  int32_t B;
+};
+
+__device__ __forceinline__ uint32_t hash32(int32_t b) {
+  uint32_t x = (uint32_t)b * 0x9E3779B1u;
+  return x ^ (x >> 15);
+}
+
+__device__ __forceinline__ int32_t ld_relaxed(int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// M[a][b]
+__device__ __forceinline__ int32_t m_get(const MView& m, int32_t a, int32_t b) {
+  const int64_t o = m.off[a];
+  const int32_t c = m.cap[a];
+  if (c == m.B) return m.val[o + b];
+  const uint32_t mask = (uint32_t)c - 1;
+  uint32_t h = hash32(b) & mask;
+  for (int32_t probe = 0; probe < c; ++probe) {
+    const int32_t k = m.key[o + h];
+    if (k == b) return m.val[o + h];
+    if (k < 0) return 0;
+    h = (h + 1) & mask;
+  }
+  return 0;
+}
+
+// M[a][b] += w (w < 0 only for a key that exists)
+__device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int32_t w,
+                                      int32_t* flags) {
+  const int64_t o = m.off[a];
+  const int32_t c = m.cap[a];
+  if (c == m.B) {
+    atomicAdd(&m.val[o + b], w);
+    return;
+  }
+  const uint32_t mask = (uint32_t)c - 1;
+  uint32_t h = hash32(b) & mask;
+  for (int32_t probe = 0; probe < c; ++probe) {
+    int32_t k = ld_relaxed(&m.key[o + h]);
+    if (k < 0) {
+      if (w < 0) {
+        atomicOr(flags, kFlagMissing);
+        return;
+      }
+      k = atomicCAS(&m.key[o + h], -1, b);
+      if (k == -1) {
+        const int32_t n = atomicAdd(&m.nkeys[a], 1) + 1;
+        if (4 * n > 3 * c) atomicOr(flags, kFlagLoad);
+        k = b;
+      }
+    }
+    if (k == b) {
+      atomicAdd(&m.val[o + h], w);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+  atomicOr(flags, w < 0 ? kFlagMissing : kFlagFull);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Orders this wavefront's earlier table writes before its later reads.
+__device__ __forceinline__ void wave_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ long wave_sum(long v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ i128 wave_sum128(i128 v) {
+  unsigned long lo = (unsigned long)(u128)v;
+  long hi = (long)(v >> 64);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const unsigned long lo2 = __shfl_xor(lo, off);
+    const long hi2 = __shfl_xor(hi, off);
+    const unsigned long s = lo + lo2;
+    hi = hi + hi2 + (s < lo ? 1 : 0);
+    lo = s;
+  }
+  return (i128)(((u128)(unsigned long)hi << 64) | (u128)lo);
+}
+
+__device__ __forceinline__ long wave_incl_scan(long v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// m * lambda(m): one L term
+__device__ __forceinline__ i128 Lterm(const int64_t* lam, int64_t m) {
+  return m > 0 ? (i128)m * (i128)lam[m] : (i128)0;
+}
+
+// m * lambda(d): one degree-factor term; inf when m > 0 and d == 0
+__device__ __forceinline__ i128 Pterm(const int64_t* lam, int64_t m, int64_t d, bool& inf) {
+  if (m <= 0) return 0;
+  if (d <= 0) {
+    inf = true;
+    return 0;
+  }
+  return (i128)m * (i128)lam[d];
+}
+
+// T(m, x, y) = m*(lambda(m)-lambda(x)-lambda(y)); inf when m > 0 and x*y == 0
+__device__ __forceinline__ i128 Tterm(const int64_t* lam, int64_t m, int64_t x, int64_t y,
+                                      bool& inf) {
+  if (m <= 0) return 0;
+  if (x <= 0 || y <= 0) {
+    inf = true;
+    return 0;
+  }
+  return (i128)m * (i128)(lam[m] - lam[x] - lam[y]);
+}
+
+// per-lane part of sum_{key <= x, key != excl} M[a][key]
+__device__ __forceinline__ long count_le_lane(const MView& m, int64_t o, int32_t c, int32_t x,
+                                              int32_t excl) {
+  const int lane = lane_id();
+  long acc = 0;
+  if (c == m.B) {
+    for (int32_t i = lane; i <= x; i += kWave)
+      if (i != excl) acc += m.val[o + i];
+  } else {
+    for (int32_t j = lane; j < c; j += kWave) {
+      const int32_t k = m.key[o + j];
+      if (k >= 0 && k <= x && k != excl) acc += m.val[o + j];
+    }
+  }
+  return acc;
+}
+
+// Inverse-CDF pick over the combined weights M[u][i] + M[i][u] (row u of M
+// and row u of M^T), ascending i, skipping i == excl: the first i whose
+// running sum exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact
+// integer weights).  The caller guarantees t < total weight.
+__device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
+                                   int32_t excl) {
+  const int lane = lane_id();
+  const int32_t B = M.B;
+  const int64_t oM = M.off[u], oT = Mt.off[u];
+  const int32_t cM = M.cap[u], cT = Mt.cap[u];
+  if (cM == B && cT == B) {
+    for (int32_t base = 0; base < B; base += kWave) {
+      const int32_t i = base + lane;
+      long w = 0;
+      if (i < B && i != excl) w = (long)M.val[oM + i] + (long)Mt.val[oT + i];
+      const long inc = wave_incl_scan(w);
+      const long tot = __shfl(inc, kWave - 1);
+      if (t < tot) {
+        const unsigned long long hit = __ballot(inc > t);
+        return base + (__ffsll((unsigned long long)hit) - 1);
+      }
+      t -= tot;
+    }
+    return B - 1;
+  }
+  int32_t lo = 0, hi = B - 1;
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    const long c = wave_sum(count_le_lane(M, oM, cM, mid, excl) + count_le_lane(Mt, oT, cT, mid, excl));
+    if (c > t)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// t-th unit of the node's neighbour list (out-neighbours then in-neighbours,
+// weight w), graph_parallel.hpp:1010-1027.
+__device__ int32_t pick_neighbor(const GraphView& g, int64_t o0, int64_t kco, int64_t i0,
+                                 int64_t kci, int64_t t) {
+  if (!g.out_w) return t < kco ? g.out_idx[o0 + t] : g.in_idx[i0 + (t - kco)];
+  const int lane = lane_id();
+  const int64_t kc = kco + kci;
+  long acc = 0;
+  for (int64_t base = 0; base < kc; base += kWave) {
+    const int64_t j = base + lane;
+    long w = 0;
+    if (j < kco)
+      w = g.out_w[o0 + j];
+    else if (j < kc)
+      w = g.in_w[i0 + (j - kco)];
+    const long inc = wave_incl_scan(w);
+    const long tot = __shfl(inc, kWave - 1);
+    if (acc + tot > t) {
+      const unsigned long long hit = __ballot(acc + inc > t);
+      const int64_t jj = base + (__ffsll((unsigned long long)hit) - 1);
+      return jj < kco ? g.out_idx[o0 + jj] : g.in_idx[i0 + (jj - kco)];
+    }
+    acc += tot;
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// M build
+// ---------------------------------------------------------------------------
+
+__global__ void k_degrees(GraphView g, const int32_t* __restrict__ labels, int32_t* d_out,
+                          int32_t* d_in, int32_t* ecnt_out, int32_t* ecnt_in, int32_t* bsize) {
+  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= g.N) return;
+  const int32_t a = labels[x];
+  const int64_t o0 = g.out_ptr[x], o1 = g.out_ptr[x + 1];
+  const int64_t i0 = g.in_ptr[x], i1 = g.in_ptr[x + 1];
+  int64_t wo = o1 - o0, wi = i1 - i0;
+  if (g.out_w) {
+    wo = 0;
+    wi = 0;
+    for (int64_t e = o0; e < o1; ++e) wo += g.out_w[e];
+    for (int64_t e = i0; e < i1; ++e) wi += g.in_w[e];
+  }
+  if (wo) atomicAdd(&d_out[a], (int32_t)wo);
+  if (wi) atomicAdd(&d_in[a], (int32_t)wi);
+  if (o1 > o0) atomicAdd(&ecnt_out[a], (int32_t)(o1 - o0));
+  if (i1 > i0) atomicAdd(&ecnt_in[a], (int32_t)(i1 - i0));
+  atomicAdd(&bsize[a], 1);
+}
+
+__device__ __forceinline__ int32_t cap_for(int32_t B, int32_t ecnt) {
+  const int64_t bound = ecnt < B ? ecnt : B;
+  int64_t c = 16;
+  while (c < 2 * bound + 8) c <<= 1;
+  return c >= B ? B : (int32_t)c;
+}
+
+__global__ void k_caps(int32_t B, const int32_t* ecnt_out, const int32_t* ecnt_in,
+                       int32_t* capM, int32_t* capT, int64_t* tmpM, int64_t* tmpT) {
+  const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a > B) return;
+  if (a == B) {
+    tmpM[B] = 0;
+    tmpT[B] = 0;
+    return;
+  }
+  const int32_t cm = cap_for(B, ecnt_out[a]);
+  const int32_t ct = cap_for(B, ecnt_in[a]);
+  capM[a] = cm;
+  capT[a] = ct;
+  tmpM[a] = cm;
+  tmpT[a] = ct;
+}
+
+__global__ void k_fill(int32_t* key, int32_t* val, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    key[i] = -1;
+    val[i] = 0;
+  }
+}
+
+__global__ void k_insert_edges(int64_t E, const int32_t* __restrict__ src,
+                               const int32_t* __restrict__ dst, const int32_t* __restrict__ w,
+                               const int32_t* __restrict__ labels, MView M, MView Mt,
+                               int32_t* flags) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t a = labels[src[e]];
+    const int32_t b = labels[dst[e]];
+    const int32_t ww = w ? w[e] : 1;
+    m_add(M, a, b, ww, flags);
+    m_add(Mt, b, a, ww, flags);
+  }
+}
+
+__global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
+  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < N) labels[x] = map[labels[x]];
+}
+
+// ---------------------------------------------------------------------------
+// Entropy, graph_parallel.hpp:1281-1319:
+// data_S = -(sum_ab L(M_ab) - sum_a L(d_out[a]) - sum_b L(d_in[b]))
+// ---------------------------------------------------------------------------
+
+__device__ void block_store128(i128 v, long* out) {
+  __shared__ long s_hi[16];
+  __shared__ unsigned long s_lo[16];
+  v = wave_sum128(v);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    s_hi[w] = (long)(v >> 64);
+    s_lo[w] = (unsigned long)(u128)v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    i128 t = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i)
+      t += (i128)(((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i]);
+    out[0] = (long)(t >> 64);
+    out[1] = (long)(unsigned long)(u128)t;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_entropy(const int32_t* __restrict__ val, int64_t nslots,
+                                                 const int32_t* __restrict__ d_out,
+                                                 const int32_t* __restrict__ d_in, int32_t B,
+                                                 const int64_t* __restrict__ lam, long* part) {
+  i128 acc = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+    const int32_t v = val[i];
+    if (v > 0) acc += (i128)v * (i128)lam[v];
+  }
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += stride)
+    acc -= Lterm(lam, d_out[b]) + Lterm(lam, d_in[b]);
+  block_store128(acc, part + 2 * blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Block merge, graph_parallel.hpp:761-988 (+ :349-368)
+// ---------------------------------------------------------------------------
+
+struct MergeArgs {
+  MView M, Mt;
+  const int32_t* d_out;
+  const int32_t* d_in;
+  const int64_t* lam;
+  int32_t B;
+  int32_t K;
+  uint64_t seed, stream;
+  int32_t* best_s;
+  long* best_hi;
+  unsigned long* best_lo;
+};
+
+// per-lane part of sum_{i not in {r,s}} [L(m1) + L(m2) - L(m1+m2)] over the
+// columns where rows r and s of m are both non-zero (the only columns whose
+// terms survive the factorised merge dS).
+__device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r, int32_t s) {
+  const int lane = lane_id();
+  const bool r_short = m.cap[r] <= m.cap[s];
+  const int32_t a1 = r_short ? r : s, a2 = r_short ? s : r;
+  const int64_t o = m.off[a1];
+  const int32_t c = m.cap[a1];
+  const bool direct = c == m.B;
+  i128 acc = 0;
+  for (int32_t j = lane; j < c; j += kWave) {
+    const int32_t i = direct ? j : m.key[o + j];
+    const int32_t m1 = m.val[o + j];
+    if (i < 0 || m1 <= 0 || i == r || i == s) continue;
+    const int32_t m2 = m_get(m, a2, i);
+    if (m2 > 0) acc += Lterm(lam, m1) + Lterm(lam, m2) - Lterm(lam, (int64_t)m1 + m2);
+  }
+  return acc;
+}
+
+__device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t& s_out,
+                          i128& dS_out) {
+  const int32_t B = a.B;
+  const int64_t dor = a.d_out[r], dir = a.d_in[r];
+  const int64_t k = dor + dir;
+  uint64_t x[4];
+  usap::philox4x64((uint64_t)r, (uint64_t)idx, a.stream, usap::kTagMerge, a.seed, 0, x);
+  int32_t s;
+  if (k == 0) {
+    s = (int32_t)usap::idx_of(x[2], B);
+  } else {
+    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1);
+    const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
+    if (usap::uniform_branch(x[1], da, B)) {
+      s = usap::uniform_not_r(x[2], r, B);
+    } else {
+      const int64_t W2 = da - m_get(a.M, u, r) - m_get(a.Mt, u, r);
+      s = W2 == 0 ? usap::uniform_not_r(x[2], r, B)
+                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r);
+    }
+  }
+  s_out = s;
+  if (s == r) {
+    dS_out = 0;
+    return;
+  }
+  const int64_t* lam = a.lam;
+  const int64_t Mrr = m_get(a.M, r, r), Mrs = m_get(a.M, r, s);
+  const int64_t Msr = m_get(a.M, s, r), Mss = m_get(a.M, s, s);
+  const int64_t dos = a.d_out[s], dis = a.d_in[s];
+  const int64_t dos2 = dos + dor, dis2 = dis + dir;
+  i128 acc = merge_overlap_lane(a.M, lam, r, s) + merge_overlap_lane(a.Mt, lam, r, s);
+  acc = wave_sum128(acc);
+  bool inf = false;
+  const i128 old_c = Tterm(lam, Mrr, dir, dor, inf) + Tterm(lam, Mrs, dis, dor, inf) +
+                     Tterm(lam, Msr, dir, dos, inf) + Tterm(lam, Mss, dis, dos, inf);
+  const i128 new_c = Tterm(lam, Mrr + Mrs + Msr + Mss, dis2, dos2, inf);
+  const int64_t Xr = dor - Mrr - Mrs, Xs = dos - Msr - Mss;
+  const int64_t Yr = dir - Mrr - Msr, Ys = dis - Mrs - Mss;
+  const i128 deg = -Pterm(lam, Xr, dor, inf) - Pterm(lam, Xs, dos, inf) +
+                   Pterm(lam, Xr + Xs, dos2, inf) - Pterm(lam, Yr, dir, inf) -
+                   Pterm(lam, Ys, dis, inf) + Pterm(lam, Yr + Ys, dis2, inf);
+  dS_out = old_c - new_c + acc + deg;
+}
+
+__global__ void k_merge(MergeArgs a) {
+  __shared__ long s_hi[kMaxMergeWaves];
+  __shared__ unsigned long s_lo[kMaxMergeWaves];
+  __shared__ int32_t s_idx[kMaxMergeWaves];
+  __shared__ int32_t s_s[kMaxMergeWaves];
+  const int32_t r = blockIdx.x;
+  const int w = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  int32_t bidx = -1, bs = -1;
+  i128 bd = 0;
+  for (int32_t idx = w; idx < a.K; idx += nw) {
+    int32_t s;
+    i128 d;
+    merge_one(a, r, idx, s, d);
+    if (bidx < 0 || d < bd) {  // strict less: the earliest proposal wins ties
+      bidx = idx;
+      bs = s;
+      bd = d;
+    }
+  }
+  if (lane_id() == 0) {
+    s_hi[w] = (long)(bd >> 64);
+    s_lo[w] = (unsigned long)(u128)bd;
+    s_idx[w] = bidx;
+    s_s[w] = bs;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int best = -1;
+    i128 bv = 0;
+    for (int i = 0; i < nw; ++i) {
+      if (s_idx[i] < 0) continue;
+      const i128 v = (i128)(((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i]);
+      if (best < 0 || v < bv || (v == bv && s_idx[i] < s_idx[best])) {
+        best = i;
+        bv = v;
+      }
+    }
+    a.best_s[r] = s_s[best];
+    a.best_hi[r] = s_hi[best];
+    a.best_lo[r] = s_lo[best];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Nodal proposal, graph_parallel.hpp:990-1222
+// ---------------------------------------------------------------------------
+
+struct NodalArgs {
+  GraphView g;
+  MView M, Mt;
+  const int32_t* labels;
+  const int32_t* d_out;
+  const int32_t* d_in;
+  const int64_t* lam;
+  int32_t B;
+  int32_t beg, end;
+  int32_t exact;
+  uint64_t seed, stream;
+  const int64_t* big_off;
+  int32_t* big_tab;
+  int32_t* prop_s;
+  long* prop_hi;
+  unsigned long* prop_lo;
+  int32_t* prop_inf;
+  int32_t* prop_acc;
+  Status* status;
+};
+
+__device__ __forceinline__ void tab_add(int32_t* tk, int32_t* tv, uint32_t mask, int32_t b,
+                                        int32_t w) {
+  uint32_t h = hash32(b) & mask;
+  for (;;) {
+    int32_t k = __hip_atomic_load(&tk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (k < 0) {
+      k = atomicCAS(&tk[h], -1, b);
+      if (k == -1) k = b;
+    }
+    if (k == b) {
+      atomicAdd(&tv[h], w);
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// dS of moving node n from r to s (r != s), factorised.  tk/two/twi is a
+// table of `cap` slots (power of two) private to this wavefront.
+__device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* twi,
+                                            int32_t cap, const NodalArgs& a, int32_t n,
+                                            int32_t r, int32_t s, int64_t o0, int64_t kco,
+                                            int64_t i0, int64_t kci, int64_t k_out,
+                                            int64_t k_in, bool& inf) {
+  const int lane = lane_id();
+  const uint32_t mask = (uint32_t)cap - 1;
+  const int64_t* lam = a.lam;
+  for (int32_t i = lane; i < cap; i += kWave) {
+    tk[i] = -1;
+    two[i] = 0;
+    twi[i] = 0;
+  }
+  wave_fence();
+  // aggregate n's edges per distinct neighbour block
+  long wself = 0;
+  for (int64_t j = lane; j < kco; j += kWave) {
+    const int32_t v = a.g.out_idx[o0 + j];
+    const int32_t ww = a.g.out_w ? a.g.out_w[o0 + j] : 1;
+    if (v == n) wself += ww;
+    tab_add(tk, two, mask, a.labels[v], ww);
+  }
+  for (int64_t j = lane; j < kci; j += kWave) {
+    const int32_t v = a.g.in_idx[i0 + j];
+    const int32_t ww = a.g.in_w ? a.g.in_w[i0 + j] : 1;
+    tab_add(tk, twi, mask, a.labels[v], ww);
+  }
+  wave_fence();
+  // changed entries of rows r, s (out-edges) and columns r, s (in-edges)
+  i128 acc = 0;
+  long Wo = 0, Wi = 0, wor = 0, wos = 0, wir = 0, wis = 0;
+  bool neg = false;
+  for (int32_t i = lane; i < cap; i += kWave) {
+    const int32_t b = tk[i];
+    if (b < 0) continue;
+    const int32_t wo = two[i], wi = twi[i];
+    if (b == r) {
+      wor = wo;
+      wir = wi;
+      continue;
+    }
+    if (b == s) {
+      wos = wo;
+      wis = wi;
+      continue;
+    }
+    if (wo > 0) {
+      const int64_t Rr = m_get(a.M, r, b), Rs = m_get(a.M, s, b);
+      neg |= Rr - wo < 0;
+      acc += Lterm(lam, Rr) + Lterm(lam, Rs) - Lterm(lam, Rr - wo) - Lterm(lam, Rs + wo);
+      Wo += wo;
+    }
+    if (wi > 0) {
+      const int64_t Cr = m_get(a.Mt, r, b), Cs = m_get(a.Mt, s, b);
+      neg |= Cr - wi < 0;
+      acc += Lterm(lam, Cr) + Lterm(lam, Cs) - Lterm(lam, Cr - wi) - Lterm(lam, Cs + wi);
+      Wi += wi;
+    }
+  }
+  if (neg) atomicOr(&a.status->flags, kFlagNegative);
+  acc = wave_sum128(acc);
+  Wo = wave_sum(Wo);
+  Wi = wave_sum(Wi);
+  wor = wave_sum(wor);
+  wos = wave_sum(wos);
+  wir = wave_sum(wir);
+  wis = wave_sum(wis);
+  wself = wave_sum(wself);
+  // corner (rows r, s at columns r, s) and the degree factors
+  const int64_t Mrr = m_get(a.M, r, r), Mrs = m_get(a.M, r, s);
+  const int64_t Msr = m_get(a.M, s, r), Mss = m_get(a.M, s, s);
+  const int64_t dor = a.d_out[r], dos = a.d_out[s], dir = a.d_in[r], dis = a.d_in[s];
+  const int64_t dor2 = dor - k_out, dos2 = dos + k_out, dir2 = dir - k_in, dis2 = dis + k_in;
+  int64_t rr2, rs2, sr2, ss2;
+  if (!a.exact) {  // graph_parallel.hpp:1117-1145: only out-edges update the rows
+    rr2 = Mrr - wor;
+    rs2 = Mrs - wos;
+    sr2 = Msr + wor;
+    ss2 = Mss + wos;
+  } else {  // graph.cu:1414-1426, with a self-loop moving (r,r) -> (s,s)
+    const int64_t wor_ = wor - wself, wir_ = wir - wself;
+    rr2 = Mrr - wor_ - wself - wir_;
+    rs2 = Mrs - wos + wir_;
+    sr2 = Msr + wor_ - wis;
+    ss2 = Mss + wos + wis + wself;
+  }
+  bool old_inf = false;
+  const i128 old_c = Tterm(lam, Mrr, dir, dor, old_inf) + Tterm(lam, Mrs, dis, dor, old_inf) +
+                     Tterm(lam, Msr, dir, dos, old_inf) + Tterm(lam, Mss, dis, dos, old_inf);
+  const i128 new_c = Tterm(lam, rr2, dir2, dor2, inf) + Tterm(lam, rs2, dis2, dor2, inf) +
+                     Tterm(lam, sr2, dir2, dos2, inf) + Tterm(lam, ss2, dis2, dos2, inf);
+  const int64_t Xr = dor - Mrr - Mrs, Xs = dos - Msr - Mss;
+  const int64_t Yr = dir - Mrr - Msr, Ys = dis - Mrs - Mss;
+  const i128 deg_old = Pterm(lam, Xr, dor, old_inf) + Pterm(lam, Xs, dos, old_inf) +
+                       Pterm(lam, Yr, dir, old_inf) + Pterm(lam, Ys, dis, old_inf);
+  const i128 deg_new = Pterm(lam, Xr - Wo, dor2, inf) + Pterm(lam, Xs + Wo, dos2, inf) +
+                       Pterm(lam, Yr - Wi, dir2, inf) + Pterm(lam, Ys + Wi, dis2, inf);
+  if (rr2 < 0 || rs2 < 0 || sr2 < 0 || ss2 < 0 || Xr - Wo < 0 || Yr - Wi < 0)
+    atomicOr(&a.status->flags, kFlagNegative);
+  return old_c - new_c + acc - deg_old + deg_new;
+}
+
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
+  __shared__ int32_t s_tab[kNodalWaves][3][kTab];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
+  const int64_t n64 = (int64_t)a.beg + item;
+  if (n64 >= a.end) return;
+  const int32_t n = (int32_t)n64;
+  const int32_t B = a.B;
+  const int32_t r = a.labels[n];
+  const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
+  const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
+  const int64_t kco = o1 - o0, kci = i1 - i0;
+  int64_t k_out = kco, k_in = kci;
+  if (a.g.out_w) {
+    long so = 0, si = 0;
+    for (int64_t j = lane; j < kco; j += kWave) so += a.g.out_w[o0 + j];
+    for (int64_t j = lane; j < kci; j += kWave) si += a.g.in_w[i0 + j];
+    k_out = wave_sum(so);
+    k_in = wave_sum(si);
+  }
+  const int64_t k = k_out + k_in;
+  uint64_t x[4];
+  usap::philox4x64((uint64_t)n, 0, a.stream, usap::kTagNodal, a.seed, 0, x);
+  int32_t s;
+  if (k == 0) {
+    s = (int32_t)usap::idx_of(x[2], B);
+  } else {
+    const int32_t v = pick_neighbor(a.g, o0, kco, i0, kci, usap::idx_of(x[0], k));
+    const int32_t u = a.labels[v];
+    const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
+    if (usap::uniform_branch(x[1], da, B))
+      s = (int32_t)usap::idx_of(x[2], B);
+    else
+      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1);
+  }
+  i128 dS = 0;
+  bool inf = false;
+  int32_t acc = 0;
+  if (s != r) {
+    if (kco + kci > kTabMaxDeg) {
+      const int64_t bo = a.big_off[n];
+      int32_t cap = 1;
+      while (cap < 2 * (kco + kci)) cap <<= 1;
+      int32_t* base = a.big_tab + bo;
+      dS = nodal_delta(base, base + cap, base + 2 * cap, cap, a, n, r, s, o0, kco, i0, kci,
+                       k_out, k_in, inf);
+    } else {
+      dS = nodal_delta(s_tab[w][0], s_tab[w][1], s_tab[w][2], kTab, a, n, r, s, o0, kco, i0,
+                       kci, k_out, k_in, inf);
+    }
+    acc = (inf || dS < 0) ? 1 : 0;
+  }
+  if (lane == 0) {
+    a.prop_s[item] = s;
+    a.prop_hi[item] = (long)(dS >> 64);
+    a.prop_lo[item] = (unsigned long)(u128)dS;
+    a.prop_inf[item] = inf ? -1 : 0;
+    a.prop_acc[item] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Apply accepted moves: M, M^T, degrees, block sizes, batch sum
+// (graph_parallel.hpp:433-443; equal to the rebuild from the new labels)
+// ---------------------------------------------------------------------------
+
+struct ApplyArgs {
+  GraphView g;
+  MView M, Mt;
+  const int32_t* labels;
+  int32_t* d_out;
+  int32_t* d_in;
+  int32_t* bsize;
+  int32_t beg, end;
+  const int32_t* prop_s;
+  const long* prop_hi;
+  const unsigned long* prop_lo;
+  const int32_t* prop_inf;
+  const int32_t* prop_acc;
+  Status* status;
+};
+
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
+  const int64_t n64 = (int64_t)a.beg + item;
+  if (n64 >= a.end) return;
+  if (!a.prop_acc[item]) return;
+  const int32_t n = (int32_t)n64;
+  const int32_t r = a.labels[n];
+  const int32_t s = a.prop_s[item];
+  int32_t* flags = &a.status->flags;
+  const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
+  const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
+  long wo_sum = 0, wi_sum = 0;
+  for (int64_t e = o0 + lane; e < o1; e += kWave) {
+    const int32_t v = a.g.out_idx[e];
+    const int32_t ww = a.g.out_w ? a.g.out_w[e] : 1;
+    const int32_t bo = a.labels[v];
+    int32_t bn = bo;
+    if (v >= a.beg && v < a.end && a.prop_acc[v - a.beg]) bn = a.prop_s[v - a.beg];
+    m_add(a.M, r, bo, -ww, flags);
+    m_add(a.Mt, bo, r, -ww, flags);
+    m_add(a.M, s, bn, ww, flags);
+    m_add(a.Mt, bn, s, ww, flags);
+    wo_sum += ww;
+  }
+  for (int64_t e = i0 + lane; e < i1; e += kWave) {
+    const int32_t v = a.g.in_idx[e];
+    const int32_t ww = a.g.in_w ? a.g.in_w[e] : 1;
+    wi_sum += ww;
+    if (v >= a.beg && v < a.end && a.prop_acc[v - a.beg]) continue;  // done as v's out-edge
+    const int32_t b = a.labels[v];
+    m_add(a.M, b, r, -ww, flags);
+    m_add(a.Mt, r, b, -ww, flags);
+    m_add(a.M, b, s, ww, flags);
+    m_add(a.Mt, s, b, ww, flags);
+  }
+  wo_sum = wave_sum(wo_sum);
+  wi_sum = wave_sum(wi_sum);
+  if (lane == 0) {
+    if (wo_sum) {
+      atomicSub(&a.d_out[r], (int32_t)wo_sum);
+      atomicAdd(&a.d_out[s], (int32_t)wo_sum);
+    }
+    if (wi_sum) {
+      atomicSub(&a.d_in[r], (int32_t)wi_sum);
+      atomicAdd(&a.d_in[s], (int32_t)wi_sum);
+    }
+    atomicSub(&a.bsize[r], 1);
+    atomicAdd(&a.bsize[s], 1);
+    atomicAdd(&a.status->moves, 1);
+    if (a.prop_inf[item]) {
+      atomicAdd(&a.status->ninf, 1);
+    } else {
+      // int128 add as two 64-bit atomics with an explicit carry: exact once
+      // every wave has added, in any order
+      const unsigned long lo = a.prop_lo[item];
+      const unsigned long long old = atomicAdd(&a.status->sum_lo, (unsigned long long)lo);
+      const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+      atomicAdd(&a.status->sum_hi, (unsigned long long)a.prop_hi[item] + carry);
+    }
+  }
+}
+
+// Also clears the status buffer of the next batch (statuses alternate, so no
+// kernel of this batch touches it).
+__global__ void k_labels(int32_t beg, int32_t end, int32_t* labels, const int32_t* prop_s,
+                         const int32_t* prop_acc, Status* next_status) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    next_status->sum_lo = 0;
+    next_status->sum_hi = 0;
+    next_status->ninf = 0;
+    next_status->moves = 0;
+    next_status->flags = 0;
+  }
+  if ((int64_t)beg + i >= end) return;
+  if (prop_acc[i]) labels[beg + i] = prop_s[i];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+struct DevSpM {
+  int64_t* off = nullptr;
+  int32_t* cap = nullptr;
+  int32_t* nkeys = nullptr;
+  int32_t* key = nullptr;
+  int32_t* val = nullptr;
+  int64_t P = 0;      // allocated slots
+  int64_t total = 0;  // slots in use
+};
+
+struct usap_engine {
+  usap_params p{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  // host copy of the graph
+  int32_t N = 0;
+  int64_t E = 0;
+  bool weighted = false;
+  std::vector<int64_t> h_out_ptr, h_in_ptr;
+  std::vector<int32_t> h_out_idx, h_in_idx, h_out_w, h_in_w;
+  usap_graph hg{};
+  int64_t Wtot = 0;
+  // device graph
+  int64_t *out_ptr = nullptr, *in_ptr = nullptr;
+  int32_t *out_idx = nullptr, *in_idx = nullptr, *out_w = nullptr, *in_w = nullptr;
+  int32_t* out_src = nullptr;
+  int64_t* lam = nullptr;
+  // partition state
+  int32_t B = 0;
+  bool has_labels = false;
+  int32_t* labels = nullptr;
+  int32_t *d_out = nullptr, *d_in = nullptr, *ecnt_out = nullptr, *ecnt_in = nullptr,
+          *bsize = nullptr;
+  DevSpM M, Mt;
+  int64_t *tmpM = nullptr, *tmpT = nullptr;
+  void* scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  // merge
+  int32_t* best_s = nullptr;
+  long* best_hi = nullptr;
+  unsigned long* best_lo = nullptr;
+  int32_t merge_B = -1;  // B of the last merge_propose (-1: none)
+  int32_t* map = nullptr;
+  // nodal
+  int32_t* prop_s = nullptr;
+  long* prop_hi = nullptr;
+  unsigned long* prop_lo = nullptr;
+  int32_t* prop_inf = nullptr;
+  int32_t* prop_acc = nullptr;
+  int32_t last_beg = 0, last_end = 0;
+  Status* status = nullptr;  // [2], alternating per batch
+  int32_t status_cur = 0;
+  Status* h_status = nullptr;
+  int32_t* build_flags = nullptr;
+  int64_t* big_off = nullptr;
+  int32_t* big_tab = nullptr;
+  // entropy
+  long* ent_part = nullptr;
+  long* h_ent_part = nullptr;
+  // snapshots
+  int32_t* snap_labels[kNumSnap] = {};
+  int32_t snap_B[kNumSnap] = {};
+  double snap_S[kNumSnap] = {};
+  int32_t rebuilds = 0;
+};
+
+namespace {
+
+int set_err(usap_engine* e, int code, const std::string& msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t _st = (expr);                                                            \
+    if (_st != hipSuccess)                                                              \
+      return set_err(e, USAP_EHIP, std::string(#expr) + ": " + hipGetErrorString(_st)); \
+  } while (0)
+
+template <typename T>
+int dalloc(usap_engine* e, T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t st = hipMalloc((void**)p, n * sizeof(T));
+  if (st != hipSuccess) {
+    *p = nullptr;
+    return set_err(e, USAP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(st));
+  }
+  return USAP_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+MView view(const DevSpM& m, int32_t B) {
+  MView v;
+  v.off = m.off;
+  v.cap = m.cap;
+  v.nkeys = m.nkeys;
+  v.key = m.key;
+  v.val = m.val;
+  v.B = B;
+  return v;
+}
+
+GraphView gview(const usap_engine* e) {
+  GraphView g;
+  g.N = e->N;
+  g.out_ptr = e->out_ptr;
+  g.out_idx = e->out_idx;
+  g.out_w = e->out_w;
+  g.in_ptr = e->in_ptr;
+  g.in_idx = e->in_idx;
+  g.in_w = e->in_w;
+  return g;
+}
+
+double fx_to_double(i128 v) {
+  const bool neg = v < 0;
+  const u128 m = neg ? (u128)0 - (u128)v : (u128)v;
+  const uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
+  const double d = std::ldexp((double)hi, 64 - usap::kLambdaFrac) +
+                   std::ldexp((double)lo, -usap::kLambdaFrac);
+  return neg ? -d : d;
+}
+
+i128 mk128(long hi, unsigned long lo) { return (i128)(((u128)(unsigned long)hi << 64) | (u128)lo); }
+
+// Full M build from the labels (graph_parallel.hpp:703-759).
+int rebuild_M(usap_engine* e) {
+  const int32_t B = e->B;
+  hipStream_t st = e->stream;
+  HIPCHK(hipMemsetAsync(e->d_out, 0, sizeof(int32_t) * B, st));
+  HIPCHK(hipMemsetAsync(e->d_in, 0, sizeof(int32_t) * B, st));
+  HIPCHK(hipMemsetAsync(e->ecnt_out, 0, sizeof(int32_t) * B, st));
+  HIPCHK(hipMemsetAsync(e->ecnt_in, 0, sizeof(int32_t) * B, st));
+  HIPCHK(hipMemsetAsync(e->bsize, 0, sizeof(int32_t) * B, st));
+  if (e->N > 0)
+    k_degrees<<<blocks_for(e->N, 256), 256, 0, st>>>(gview(e), e->labels, e->d_out, e->d_in,
+                                                      e->ecnt_out, e->ecnt_in, e->bsize);
+  k_caps<<<blocks_for((int64_t)B + 1, 256), 256, 0, st>>>(B, e->ecnt_out, e->ecnt_in, e->M.cap,
+                                                          e->Mt.cap, e->tmpM, e->tmpT);
+  HIPCHK(hipGetLastError());
+  size_t bytes = 0;
+  HIPCHK(rocprim::exclusive_scan(nullptr, bytes, e->tmpM, e->M.off, (int64_t)0, (size_t)B + 1,
+                                 rocprim::plus<int64_t>(), st));
+  if (bytes > e->scan_tmp_bytes) {
+    dfree(e->scan_tmp);
+    HIPCHK(hipMalloc(&e->scan_tmp, bytes));
+    e->scan_tmp_bytes = bytes;
+  }
+  bytes = e->scan_tmp_bytes;
+  HIPCHK(rocprim::exclusive_scan(e->scan_tmp, bytes, e->tmpM, e->M.off, (int64_t)0, (size_t)B + 1,
+                                 rocprim::plus<int64_t>(), st));
+  bytes = e->scan_tmp_bytes;
+  HIPCHK(rocprim::exclusive_scan(e->scan_tmp, bytes, e->tmpT, e->Mt.off, (int64_t)0, (size_t)B + 1,
+                                 rocprim::plus<int64_t>(), st));
+  int64_t tot[2];
+  HIPCHK(hipMemcpyAsync(&tot[0], e->M.off + B, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&tot[1], e->Mt.off + B, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  DevSpM* ms[2] = {&e->M, &e->Mt};
+  for (int i = 0; i < 2; ++i) {
+    DevSpM& m = *ms[i];
+    m.total = tot[i];
+    if (m.total > m.P) {
+      dfree(m.key);
+      dfree(m.val);
+      const int64_t P = m.total + m.total / 4 + 1024;
+      if (int rc = dalloc(e, &m.key, (size_t)P)) return rc;
+      if (int rc = dalloc(e, &m.val, (size_t)P)) return rc;
+      m.P = P;
+    }
+    if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
+        m.key, m.val, m.total);
+    HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
+  }
+  HIPCHK(hipMemsetAsync(e->build_flags, 0, sizeof(int32_t), st));
+  if (e->E > 0)
+    k_insert_edges<<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
+        e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), view(e->Mt, B),
+        e->build_flags);
+  HIPCHK(hipGetLastError());
+  int32_t flags = 0;
+  HIPCHK(hipMemcpyAsync(&flags, e->build_flags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (flags & (kFlagFull | kFlagMissing))
+    return set_err(e, USAP_ESTATE, "M build overflowed a row slab");
+  e->rebuilds++;
+  return USAP_OK;
+}
+
+int check_engine(usap_engine* e) {
+  if (!e) return USAP_EINVAL;
+  if (!e->has_labels) return set_err(e, USAP_ESTATE, "labels not set");
+  return USAP_OK;
+}
+
+}  // namespace
+
+namespace usap {
+const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
+const usap_params* engine_params(const usap_engine* e) { return &e->p; }
+int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
+void engine_set_error(usap_engine* e, const char* msg) { e->err = msg; }
+}  // namespace usap
+
+extern "C" {
+
+void usap_default_params(usap_params* p) {
+  p->K = 10;
+  p->reduction_rate = 0.5;
+  p->max_nodal_itr = 100;
+  p->num_batch_nodal_update = 4;
+  p->nodal_batch = 0;
+  p->thr1 = 5e-4;
+  p->thr2 = 1e-4;
+  p->avg_window = 3;
+  p->dfs_depth = 20;
+  p->init = USAP_INIT_SCC;
+  p->ds_mode = USAP_DS_REF;
+  p->seed = 0;
+  p->device = 0;
+  p->verbose = 0;
+}
+
+const char* usap_last_error(const usap_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int usap_destroy(usap_engine* e) {
+  if (!e) return USAP_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  dfree(e->out_ptr);
+  dfree(e->in_ptr);
+  dfree(e->out_idx);
+  dfree(e->in_idx);
+  dfree(e->out_w);
+  dfree(e->in_w);
+  dfree(e->out_src);
+  dfree(e->lam);
+  dfree(e->labels);
+  dfree(e->d_out);
+  dfree(e->d_in);
+  dfree(e->ecnt_out);
+  dfree(e->ecnt_in);
+  dfree(e->bsize);
+  for (DevSpM* m : {&e->M, &e->Mt}) {
+    dfree(m->off);
+    dfree(m->cap);
+    dfree(m->nkeys);
+    dfree(m->key);
+    dfree(m->val);
+  }
+  dfree(e->tmpM);
+  dfree(e->tmpT);
+  if (e->scan_tmp) (void)hipFree(e->scan_tmp);
+  dfree(e->best_s);
+  dfree(e->best_hi);
+  dfree(e->best_lo);
+  dfree(e->map);
+  dfree(e->prop_s);
+  dfree(e->prop_hi);
+  dfree(e->prop_lo);
+  dfree(e->prop_inf);
+  dfree(e->prop_acc);
+  dfree(e->status);
+  dfree(e->build_flags);
+  if (e->h_status) (void)hipHostFree(e->h_status);
+  dfree(e->big_off);
+  dfree(e->big_tab);
+  dfree(e->ent_part);
+  if (e->h_ent_part) (void)hipHostFree(e->h_ent_part);
+  for (int i = 0; i < kNumSnap; ++i) dfree(e->snap_labels[i]);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return USAP_OK;
+}
+
+int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
+  if (!g || !out) return USAP_EINVAL;
+  *out = nullptr;
+  if (g->N < 1 || g->E < 0 || !g->out_ptr || !g->in_ptr || (g->E > 0 && (!g->out_idx || !g->in_idx)))
+    return USAP_EINVAL;
+  if ((g->out_w == nullptr) != (g->in_w == nullptr)) return USAP_EINVAL;
+  const int32_t N = g->N;
+  const int64_t E = g->E;
+  if (g->out_ptr[0] != 0 || g->out_ptr[N] != E || g->in_ptr[0] != 0 || g->in_ptr[N] != E)
+    return USAP_EINVAL;
+  usap_engine* e = new (std::nothrow) usap_engine();
+  if (!e) return USAP_ENOMEM;
+  if (p)
+    e->p = *p;
+  else
+    usap_default_params(&e->p);
+  if (e->p.K < 1 || e->p.reduction_rate <= 0 || e->p.reduction_rate >= 1 || e->p.avg_window < 2 ||
+      e->p.max_nodal_itr < 1) {
+    delete e;
+    return USAP_EINVAL;
+  }
+  e->N = N;
+  e->E = E;
+  e->weighted = g->out_w != nullptr;
+  e->h_out_ptr.assign(g->out_ptr, g->out_ptr + N + 1);
+  e->h_in_ptr.assign(g->in_ptr, g->in_ptr + N + 1);
+  e->h_out_idx.assign(g->out_idx, g->out_idx + E);
+  e->h_in_idx.assign(g->in_idx, g->in_idx + E);
+  int64_t wtot = E;
+  if (e->weighted) {
+    e->h_out_w.assign(g->out_w, g->out_w + E);
+    e->h_in_w.assign(g->in_w, g->in_w + E);
+    wtot = 0;
+    for (int64_t i = 0; i < E; ++i) {
+      if (g->out_w[i] < 0) {
+        delete e;
+        return USAP_EINVAL;
+      }
+      wtot += g->out_w[i];
+    }
+  }
+  for (int64_t i = 0; i < E; ++i)
+    if (g->out_idx[i] < 0 || g->out_idx[i] >= N || g->in_idx[i] < 0 || g->in_idx[i] >= N) {
+      delete e;
+      return USAP_EINVAL;
+    }
+  if (wtot >= (int64_t)1 << 31) {  // M counts and degrees are int32
+    delete e;
+    return USAP_EINVAL;
+  }
+  e->Wtot = wtot;
+  e->hg.N = N;
+  e->hg.E = E;
+  e->hg.out_ptr = e->h_out_ptr.data();
+  e->hg.in_ptr = e->h_in_ptr.data();
+  e->hg.out_idx = e->h_out_idx.data();
+  e->hg.in_idx = e->h_in_idx.data();
+  e->hg.out_w = e->weighted ? e->h_out_w.data() : nullptr;
+  e->hg.in_w = e->weighted ? e->h_in_w.data() : nullptr;
+
+  int rc = USAP_OK;
+  hipError_t hst = hipSetDevice(e->p.device);
+  if (hst != hipSuccess) {
+    rc = set_err(e, USAP_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(hst));
+    usap_destroy(e);
+    return rc;
+  }
+  hst = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (hst != hipSuccess) {
+    e->stream = nullptr;
+    usap_destroy(e);
+    return USAP_EHIP;
+  }
+#define TRY(x)            \
+  do {                    \
+    rc = (x);             \
+    if (rc) goto fail;    \
+  } while (0)
+#define TRYH(x)                                                                        \
+  do {                                                                                 \
+    hipError_t _h = (x);                                                               \
+    if (_h != hipSuccess) {                                                            \
+      rc = set_err(e, USAP_EHIP, std::string(#x) + ": " + hipGetErrorString(_h));      \
+      goto fail;                                                                       \
+    }                                                                                  \
+  } while (0)
+  {
+    TRY(dalloc(e, &e->out_ptr, N + 1));
+    TRY(dalloc(e, &e->in_ptr, N + 1));
+    TRY(dalloc(e, &e->out_idx, E));
+    TRY(dalloc(e, &e->in_idx, E));
+    TRY(dalloc(e, &e->out_src, E));
+    if (e->weighted) {
+      TRY(dalloc(e, &e->out_w, E));
+      TRY(dalloc(e, &e->in_w, E));
+    }
+    TRY(dalloc(e, &e->lam, wtot + 1));
+    TRY(dalloc(e, &e->labels, N));
+    TRY(dalloc(e, &e->d_out, N));
+    TRY(dalloc(e, &e->d_in, N));
+    TRY(dalloc(e, &e->ecnt_out, N));
+    TRY(dalloc(e, &e->ecnt_in, N));
+    TRY(dalloc(e, &e->bsize, N));
+    for (DevSpM* m : {&e->M, &e->Mt}) {
+      TRY(dalloc(e, &m->off, (size_t)N + 1));
+      TRY(dalloc(e, &m->cap, N));
+      TRY(dalloc(e, &m->nkeys, N));
+    }
+    TRY(dalloc(e, &e->tmpM, (size_t)N + 1));
+    TRY(dalloc(e, &e->tmpT, (size_t)N + 1));
+    TRY(dalloc(e, &e->best_s, N));
+    TRY(dalloc(e, &e->best_hi, N));
+    TRY(dalloc(e, &e->best_lo, N));
+    TRY(dalloc(e, &e->map, N));
+    TRY(dalloc(e, &e->prop_s, N));
+    TRY(dalloc(e, &e->prop_hi, N));
+    TRY(dalloc(e, &e->prop_lo, N));
+    TRY(dalloc(e, &e->prop_inf, N));
+    TRY(dalloc(e, &e->prop_acc, N));
+    TRY(dalloc(e, &e->status, 2));
+    TRY(dalloc(e, &e->build_flags, 1));
+    TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
+    if (hipHostMalloc((void**)&e->h_status, sizeof(Status)) != hipSuccess ||
+        hipHostMalloc((void**)&e->h_ent_part, sizeof(long) * 2 * kEntropyBlocks) != hipSuccess) {
+      rc = set_err(e, USAP_ENOMEM, "hipHostMalloc");
+      goto fail;
+    }
+    // lambda table
+    std::vector<int64_t> lam((size_t)wtot + 1);
+    usap_lambda_table(wtot, lam.data());
+    std::vector<int32_t> src((size_t)E);
+    for (int32_t x = 0; x < N; ++x)
+      for (int64_t j = g->out_ptr[x]; j < g->out_ptr[x + 1]; ++j) src[j] = x;
+    // global-memory neighbour tables of the nodes too large for LDS
+    std::vector<int64_t> big((size_t)N, -1);
+    int64_t big_total = 0;
+    for (int32_t x = 0; x < N; ++x) {
+      const int64_t k = (g->out_ptr[x + 1] - g->out_ptr[x]) + (g->in_ptr[x + 1] - g->in_ptr[x]);
+      if (k > kTabMaxDeg) {
+        int64_t cap = 1;
+        while (cap < 2 * k) cap <<= 1;
+        big[x] = big_total;
+        big_total += 3 * cap;
+      }
+    }
+    TRY(dalloc(e, &e->big_off, N));
+    TRY(dalloc(e, &e->big_tab, big_total));
+    hipStream_t st = e->stream;
+    TRYH(hipMemcpyAsync(e->out_ptr, g->out_ptr, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
+    TRYH(hipMemcpyAsync(e->in_ptr, g->in_ptr, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
+    if (E > 0) {
+      TRYH(hipMemcpyAsync(e->out_idx, g->out_idx, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+      TRYH(hipMemcpyAsync(e->in_idx, g->in_idx, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+      TRYH(hipMemcpyAsync(e->out_src, src.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+      if (e->weighted) {
+        TRYH(hipMemcpyAsync(e->out_w, g->out_w, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+        TRYH(hipMemcpyAsync(e->in_w, g->in_w, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+      }
+    }
+    TRYH(hipMemcpyAsync(e->lam, lam.data(), sizeof(int64_t) * (wtot + 1), hipMemcpyHostToDevice, st));
+    TRYH(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, st));
+    TRYH(hipMemsetAsync(e->status, 0, 2 * sizeof(Status), st));
+    TRYH(hipStreamSynchronize(st));
+  }
+#undef TRY
+#undef TRYH
+  *out = e;
+  return USAP_OK;
+fail:
+  usap_destroy(e);
+  return rc;
+}
+
+int usap_set_labels(usap_engine* e, const int32_t* labels, int32_t B) {
+  if (!e || !labels || B < 1 || B > e->N) return USAP_EINVAL;
+  for (int32_t x = 0; x < e->N; ++x)
+    if (labels[x] < 0 || labels[x] >= B) return set_err(e, USAP_EINVAL, "label out of range");
+  HIPCHK(hipMemcpyAsync(e->labels, labels, sizeof(int32_t) * e->N, hipMemcpyHostToDevice, e->stream));
+  e->B = B;
+  e->has_labels = true;
+  e->merge_B = -1;
+  return rebuild_M(e);
+}
+
+int usap_get_labels(usap_engine* e, int32_t* out) {
+  if (int rc = check_engine(e)) return rc;
+  if (!out) return USAP_EINVAL;
+  HIPCHK(hipMemcpyAsync(out, e->labels, sizeof(int32_t) * e->N, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return USAP_OK;
+}
+
+int usap_num_blocks(usap_engine* e, int32_t* B) {
+  if (int rc = check_engine(e)) return rc;
+  if (!B) return USAP_EINVAL;
+  *B = e->B;
+  return USAP_OK;
+}
+
+int usap_get_degrees(usap_engine* e, int32_t* d_out, int32_t* d_in) {
+  if (int rc = check_engine(e)) return rc;
+  if (d_out)
+    HIPCHK(hipMemcpyAsync(d_out, e->d_out, sizeof(int32_t) * e->B, hipMemcpyDeviceToHost, e->stream));
+  if (d_in)
+    HIPCHK(hipMemcpyAsync(d_in, e->d_in, sizeof(int32_t) * e->B, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return USAP_OK;
+}
+
+int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, int32_t* vals,
+                   int64_t* nnz) {
+  if (int rc = check_engine(e)) return rc;
+  if (!nnz) return USAP_EINVAL;
+  const int32_t B = e->B;
+  std::vector<int64_t> off((size_t)B + 1);
+  std::vector<int32_t> capv((size_t)B);
+  std::vector<int32_t> key((size_t)e->M.total), val((size_t)e->M.total);
+  hipStream_t st = e->stream;
+  HIPCHK(hipMemcpyAsync(off.data(), e->M.off, sizeof(int64_t) * (B + 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(capv.data(), e->M.cap, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+  if (e->M.total > 0) {
+    HIPCHK(hipMemcpyAsync(key.data(), e->M.key, sizeof(int32_t) * e->M.total, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(val.data(), e->M.val, sizeof(int32_t) * e->M.total, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<std::pair<int32_t, int32_t>> row;
+  int64_t k = 0;
+  for (int32_t a = 0; a < B; ++a) {
+    row.clear();
+    for (int64_t j = 0; j < capv[a]; ++j) {
+      const int32_t c = capv[a] == B ? (int32_t)j : key[off[a] + j];
+      const int32_t v = val[off[a] + j];
+      if (c >= 0 && v != 0) row.emplace_back(c, v);
+    }
+    std::sort(row.begin(), row.end());
+    for (auto& cv : row) {
+      if (k < cap && rows && cols && vals) {
+        rows[k] = a;
+        cols[k] = cv.first;
+        vals[k] = cv.second;
+      }
+      ++k;
+    }
+  }
+  *nnz = k;
+  return USAP_OK;
+}
+
+int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
+  if (int rc = check_engine(e)) return rc;
+  hipStream_t st = e->stream;
+  k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.val, e->M.total, e->d_out, e->d_in, e->B, e->lam,
+                                           e->ent_part);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(e->h_ent_part, e->ent_part, sizeof(long) * 2 * kEntropyBlocks,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  i128 t = 0;
+  for (int i = 0; i < kEntropyBlocks; ++i)
+    t += mk128(e->h_ent_part[2 * i], (unsigned long)e->h_ent_part[2 * i + 1]);
+  const i128 data = -t;
+  if (hi) *hi = (int64_t)(data >> 64);
+  if (lo) *lo = (uint64_t)(u128)data;
+  if (S) *S = usap_model_S(e->N, e->E, e->B) + fx_to_double(data);
+  return USAP_OK;
+}
+
+int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best_s,
+                       double* best_dS) {
+  if (int rc = check_engine(e)) return rc;
+  if (K < 1) return USAP_EINVAL;
+  const int32_t B = e->B;
+  MergeArgs a;
+  a.M = view(e->M, B);
+  a.Mt = view(e->Mt, B);
+  a.d_out = e->d_out;
+  a.d_in = e->d_in;
+  a.lam = e->lam;
+  a.B = B;
+  a.K = K;
+  a.seed = e->p.seed;
+  a.stream = stream;
+  a.best_s = e->best_s;
+  a.best_hi = e->best_hi;
+  a.best_lo = e->best_lo;
+  const int nw = std::min(K, kMaxMergeWaves);
+  k_merge<<<B, nw * kWave, 0, e->stream>>>(a);
+  HIPCHK(hipGetLastError());
+  e->merge_B = B;
+  if (best_s || best_dS) {
+    std::vector<long> h((size_t)B);
+    std::vector<unsigned long> l((size_t)B);
+    if (best_s)
+      HIPCHK(hipMemcpyAsync(best_s, e->best_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost, e->stream));
+    if (best_dS) {
+      HIPCHK(hipMemcpyAsync(h.data(), e->best_hi, sizeof(long) * B, hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(l.data(), e->best_lo, sizeof(long) * B, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (best_dS)
+      for (int32_t r = 0; r < B; ++r) best_dS[r] = fx_to_double(mk128(h[r], l[r]));
+  } else {
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  return USAP_OK;
+}
+
+int usap_merge_result_fx(usap_engine* e, int32_t* best_s, int64_t* hi, uint64_t* lo) {
+  if (int rc = check_engine(e)) return rc;
+  if (e->merge_B != e->B) return set_err(e, USAP_ESTATE, "no merge proposals for this state");
+  const int32_t B = e->B;
+  if (best_s) HIPCHK(hipMemcpyAsync(best_s, e->best_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost, e->stream));
+  if (hi) HIPCHK(hipMemcpyAsync(hi, e->best_hi, sizeof(int64_t) * B, hipMemcpyDeviceToHost, e->stream));
+  if (lo) HIPCHK(hipMemcpyAsync(lo, e->best_lo, sizeof(uint64_t) * B, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return USAP_OK;
+}
+
+int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new) {
+  if (int rc = check_engine(e)) return rc;
+  const int32_t B = e->B;
+  if (e->merge_B != B) return set_err(e, USAP_ESTATE, "merge_apply without merge_propose");
+  if (B_to_merge < 0 || B_to_merge >= B) return set_err(e, USAP_EINVAL, "B_to_merge out of range");
+  std::vector<int32_t> best((size_t)B), bsize((size_t)B);
+  std::vector<long> h((size_t)B);
+  std::vector<unsigned long> l((size_t)B);
+  hipStream_t st = e->stream;
+  HIPCHK(hipMemcpyAsync(best.data(), e->best_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(h.data(), e->best_hi, sizeof(long) * B, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(l.data(), e->best_lo, sizeof(long) * B, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(bsize.data(), e->bsize, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  // order by (dS, block id): the reference sorts by dS with an unstable sort
+  std::vector<int32_t> order((size_t)B);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int32_t i, int32_t j) {
+    const i128 a = mk128(h[i], l[i]), b = mk128(h[j], l[j]);
+    return a < b || (a == b && i < j);
+  });
+  // greedy merges: block_map of graph_parallel.hpp:1236-1257 as a union-find
+  // whose root is the representative (the merge target's current block)
+  std::vector<int32_t> parent((size_t)B);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int32_t x) {
+    int32_t r = x;
+    while (parent[r] != r) r = parent[r];
+    while (parent[x] != r) {
+      const int32_t n = parent[x];
+      parent[x] = r;
+      x = n;
+    }
+    return r;
+  };
+  int32_t num = 0, counter = 0;
+  while (num < B_to_merge) {
+    if (counter >= B) return set_err(e, USAP_EINFEASIBLE, "carry-out ran out of candidate merges");
+    const int32_t from = order[counter++];
+    const int32_t to = find(best[from]);
+    if (to != from) {
+      parent[from] = to;
+      ++num;
+    }
+  }
+  // compaction: surviving labels ranked ascending (graph_parallel.hpp:1260-1277)
+  std::vector<char> alive((size_t)B, 0);
+  for (int32_t b = 0; b < B; ++b)
+    if (bsize[b] > 0) alive[find(b)] = 1;
+  std::vector<int32_t> rank((size_t)B, -1);
+  int32_t next = 0;
+  for (int32_t b = 0; b < B; ++b)
+    if (alive[b]) rank[b] = next++;
+  std::vector<int32_t> map((size_t)B);
+  for (int32_t b = 0; b < B; ++b) map[b] = rank[find(b)] < 0 ? 0 : rank[find(b)];
+  HIPCHK(hipMemcpyAsync(e->map, map.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+  k_relabel<<<blocks_for(e->N, 256), 256, 0, st>>>(e->N, e->labels, e->map);
+  HIPCHK(hipGetLastError());
+  e->B = B - B_to_merge;
+  e->merge_B = -1;
+  if (B_new) *B_new = e->B;
+  return rebuild_M(e);
+}
+
+int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream, double* dS_sum,
+                     int32_t* n_moves) {
+  if (int rc = check_engine(e)) return rc;
+  if (begin < 0 || end > e->N || begin > end) return USAP_EINVAL;
+  hipStream_t st = e->stream;
+  e->last_beg = begin;
+  e->last_end = end;
+  if (begin == end) {
+    std::memset(e->h_status, 0, sizeof(Status));
+    if (dS_sum) *dS_sum = 0;
+    if (n_moves) *n_moves = 0;
+    return USAP_OK;
+  }
+  const int32_t B = e->B;
+  const int64_t n = end - begin;
+  NodalArgs a;
+  a.g = gview(e);
+  a.M = view(e->M, B);
+  a.Mt = view(e->Mt, B);
+  a.labels = e->labels;
+  a.d_out = e->d_out;
+  a.d_in = e->d_in;
+  a.lam = e->lam;
+  a.B = B;
+  a.beg = begin;
+  a.end = end;
+  a.exact = e->p.ds_mode == USAP_DS_EXACT;
+  a.seed = e->p.seed;
+  a.stream = stream;
+  a.big_off = e->big_off;
+  a.big_tab = e->big_tab;
+  a.prop_s = e->prop_s;
+  a.prop_hi = e->prop_hi;
+  a.prop_lo = e->prop_lo;
+  a.prop_inf = e->prop_inf;
+  a.prop_acc = e->prop_acc;
+  Status* cur = e->status + e->status_cur;
+  Status* nxt = e->status + (e->status_cur ^ 1);
+  a.status = cur;
+  const unsigned grid = blocks_for(n, kNodalWaves);
+  k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
+  ApplyArgs b;
+  b.g = a.g;
+  b.M = a.M;
+  b.Mt = a.Mt;
+  b.labels = e->labels;
+  b.d_out = e->d_out;
+  b.d_in = e->d_in;
+  b.bsize = e->bsize;
+  b.beg = begin;
+  b.end = end;
+  b.prop_s = e->prop_s;
+  b.prop_hi = e->prop_hi;
+  b.prop_lo = e->prop_lo;
+  b.prop_inf = e->prop_inf;
+  b.prop_acc = e->prop_acc;
+  b.status = cur;
+  k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
+  k_labels<<<blocks_for(n, 256), 256, 0, st>>>(begin, end, e->labels, e->prop_s, e->prop_acc, nxt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(e->h_status, cur, sizeof(Status), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  e->status_cur ^= 1;
+  const Status& s = *e->h_status;
+  if (s.flags & (kFlagMissing | kFlagNegative))
+    return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
+  if (s.flags & (kFlagLoad | kFlagFull)) {
+    if (int rc = rebuild_M(e)) return rc;
+  }
+  if (dS_sum) {
+    if (s.ninf > 0)
+      *dS_sum = -INFINITY;
+    else
+      *dS_sum = fx_to_double(mk128((long)s.sum_hi, (unsigned long)s.sum_lo));
+  }
+  if (n_moves) *n_moves = s.moves;
+  return USAP_OK;
+}
+
+int usap_nodal_sum_fx(usap_engine* e, int64_t* hi, uint64_t* lo, int32_t* inf) {
+  if (int rc = check_engine(e)) return rc;
+  const Status& s = *e->h_status;
+  if (hi) *hi = (int64_t)s.sum_hi;
+  if (lo) *lo = (uint64_t)s.sum_lo;
+  if (inf) *inf = s.ninf > 0 ? -1 : 0;
+  return USAP_OK;
+}
+
+int usap_nodal_detail(usap_engine* e, int32_t* s, int64_t* hi, uint64_t* lo, int32_t* inf,
+                      int32_t* accept) {
+  if (int rc = check_engine(e)) return rc;
+  const int64_t n = e->last_end - e->last_beg;
+  hipStream_t st = e->stream;
+  if (n > 0) {
+    if (s) HIPCHK(hipMemcpyAsync(s, e->prop_s, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    if (hi) HIPCHK(hipMemcpyAsync(hi, e->prop_hi, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+    if (lo) HIPCHK(hipMemcpyAsync(lo, e->prop_lo, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, st));
+    if (inf) HIPCHK(hipMemcpyAsync(inf, e->prop_inf, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    if (accept) HIPCHK(hipMemcpyAsync(accept, e->prop_acc, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return USAP_OK;
+}
+
+int usap_snapshot(usap_engine* e, int32_t slot, double S) {
+  if (int rc = check_engine(e)) return rc;
+  if (slot < 0 || slot >= kNumSnap) return USAP_EINVAL;
+  if (!e->snap_labels[slot])
+    if (int rc = dalloc(e, &e->snap_labels[slot], e->N)) return rc;
+  HIPCHK(hipMemcpyAsync(e->snap_labels[slot], e->labels, sizeof(int32_t) * e->N,
+                        hipMemcpyDeviceToDevice, e->stream));
+  e->snap_B[slot] = e->B;
+  e->snap_S[slot] = S;
+  return USAP_OK;
+}
+
+int usap_restore(usap_engine* e, int32_t slot) {
+  if (int rc = check_engine(e)) return rc;
+  if (slot < 0 || slot >= kNumSnap || !e->snap_labels[slot]) return USAP_EINVAL;
+  HIPCHK(hipMemcpyAsync(e->labels, e->snap_labels[slot], sizeof(int32_t) * e->N,
+                        hipMemcpyDeviceToDevice, e->stream));
+  e->B = e->snap_B[slot];
+  e->merge_B = -1;
+  return rebuild_M(e);
+}
+
+}  // extern "C"

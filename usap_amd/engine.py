@@ -1,0 +1,181 @@
+"""Python handle over one usap_engine (include/usap.h)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import UsapParams, UsapStats, check, lib, ptr
+
+FRAC = 48
+INIT_SCC, INIT_IDENTITY = 0, 1
+DS_REF, DS_EXACT = 0, 1
+
+
+def default_params(**kw):
+    p = UsapParams()
+    lib().usap_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def fx(hi, lo):
+    """Exact int128 (scale 2^-48) from its (hi, lo) words."""
+    return (int(hi) << 64) | int(np.uint64(lo))
+
+
+class Engine:
+    """One graph resident on the device: labels, M (+ M^T), block degrees."""
+
+    def __init__(self, graph, **params):
+        self.graph = graph
+        self.params = default_params(**params)
+        self._gs = graph.as_struct()
+        h = C.c_void_p()
+        check(lib().usap_create(C.byref(self._gs), C.byref(self.params), C.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().usap_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc):
+        return check(rc, self.h)
+
+    # ---- state ----------------------------------------------------------
+    def set_labels(self, labels, B):
+        self._ck(lib().usap_set_labels(self.h, np.ascontiguousarray(labels, np.int32), int(B)))
+
+    def labels(self):
+        out = np.zeros(self.graph.N, np.int32)
+        self._ck(lib().usap_get_labels(self.h, out))
+        return out
+
+    @property
+    def B(self):
+        b = C.c_int32()
+        self._ck(lib().usap_num_blocks(self.h, C.byref(b)))
+        return b.value
+
+    def degrees(self):
+        B = self.B
+        do, di = np.zeros(B, np.int32), np.zeros(B, np.int32)
+        self._ck(lib().usap_get_degrees(self.h, do, di))
+        return do, di
+
+    def M_coo(self):
+        nnz = C.c_int64()
+        self._ck(lib().usap_get_M_coo(self.h, 0, None, None, None, C.byref(nnz)))
+        n = nnz.value
+        r, c, v = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+        self._ck(lib().usap_get_M_coo(self.h, n, ptr(r), ptr(c), ptr(v), C.byref(nnz)))
+        return r[:n], c[:n], v[:n]
+
+    def M_dense(self):
+        B = self.B
+        r, c, v = self.M_coo()
+        M = np.zeros((B, B), np.int64)
+        M[r, c] = v
+        return M
+
+    def entropy(self):
+        S, hi, lo = C.c_double(), C.c_int64(), C.c_uint64()
+        self._ck(lib().usap_entropy(self.h, C.byref(S), C.byref(hi), C.byref(lo)))
+        return fx(hi.value, lo.value), S.value
+
+    # ---- hot path -------------------------------------------------------
+    def merge_propose(self, stream, K=None):
+        K = self.params.K if K is None else K
+        self._ck(lib().usap_merge_propose(self.h, K, stream, None, None))
+        B = self.B
+        s = np.zeros(B, np.int32)
+        hi = np.zeros(B, np.int64)
+        lo = np.zeros(B, np.uint64)
+        self._ck(lib().usap_merge_result_fx(self.h, s, hi, lo))
+        return s, [fx(h, l) for h, l in zip(hi, lo)]
+
+    def merge_apply(self, B_to_merge):
+        b = C.c_int32()
+        self._ck(lib().usap_merge_apply(self.h, int(B_to_merge), C.byref(b)))
+        return b.value
+
+    def nodal_batch(self, beg, end, stream, detail=True):
+        dS, mv = C.c_double(), C.c_int32()
+        self._ck(lib().usap_nodal_batch(self.h, beg, end, stream, C.byref(dS), C.byref(mv)))
+        hi, lo, inf = C.c_int64(), C.c_uint64(), C.c_int32()
+        self._ck(lib().usap_nodal_sum_fx(self.h, C.byref(hi), C.byref(lo), C.byref(inf)))
+        out = dict(sum=fx(hi.value, lo.value), sum_inf=inf.value, moves=mv.value,
+                   dS_sum=dS.value)
+        if detail:
+            n = end - beg
+            s = np.zeros(max(n, 1), np.int32)
+            h = np.zeros(max(n, 1), np.int64)
+            l = np.zeros(max(n, 1), np.uint64)
+            fi = np.zeros(max(n, 1), np.int32)
+            ac = np.zeros(max(n, 1), np.int32)
+            self._ck(lib().usap_nodal_detail(self.h, s, h, l, fi, ac))
+            out.update(s=s[:n], dS=[fx(a, b) for a, b in zip(h[:n], l[:n])], inf=fi[:n],
+                       accept=ac[:n])
+        return out
+
+    def snapshot(self, slot, S=0.0):
+        self._ck(lib().usap_snapshot(self.h, slot, S))
+
+    def restore(self, slot):
+        self._ck(lib().usap_restore(self.h, slot))
+
+    def partition(self):
+        """Full partition() through the host C++ driver.  Returns (labels, stats)."""
+        lab = np.zeros(self.graph.N, np.int32)
+        st = UsapStats()
+        self._ck(lib().usap_partition(self.h, lab.ctypes.data, C.byref(st)))
+        return lab, st.as_dict()
+
+
+def scc_init(graph, depth=20):
+    """SCC initial partition (graph_parallel.hpp:514-603): (labels, B)."""
+    gs = graph.as_struct()
+    lab = np.zeros(graph.N, np.int32)
+    B = C.c_int32()
+    check(lib().usap_scc_init(C.byref(gs), depth, lab, C.byref(B)))
+    return lab, B.value
+
+
+def philox4x64(ctr, key):
+    out = np.zeros(4, np.uint64)
+    lib().usap_philox4x64(np.asarray(ctr, np.uint64), np.asarray(key, np.uint64), out)
+    return out
+
+
+def lambda_table(maxv):
+    out = np.zeros(maxv + 1, np.int64)
+    check(lib().usap_lambda_table(maxv, out))
+    return out
+
+
+def model_S(N, E, B):
+    return lib().usap_model_S(N, E, B)
+
+
+def golden_update(slots, cur, rate=0.5):
+    """slots: [(B, S, id)] * 3 (large, med, small); cur: (B, S, id).
+    Returns (rc, slots, next, B_to_merge)."""
+    Bs = np.array([s[0] for s in slots], np.int64)
+    Ss = np.array([s[1] for s in slots], np.float64)
+    ids = np.array([s[2] for s in slots], np.int64)
+    nB, nS, nid, btm = C.c_int64(), C.c_double(), C.c_int64(), C.c_int64()
+    rc = lib().usap_golden_update(Bs, Ss, ids, cur[0], cur[1], cur[2], rate, C.byref(nB),
+                                  C.byref(nS), C.byref(nid), C.byref(btm))
+    return rc, list(zip(Bs.tolist(), Ss.tolist(), ids.tolist())), (nB.value, nS.value,
+                                                                    nid.value), btm.value
