@@ -1,0 +1,412 @@
+"""Pins the CPU oracle (oracle/usap_oracle.c) against independent checks:
+numpy's Philox, high-precision logarithms, brute-force entropy differences,
+exact enumeration of the samplers, hand-derived SCC and golden-section cases.
+"""
+import math
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+from conftest import random_graph
+
+FRAC = 1 << 48
+
+
+# ---------------------------------------------------------------------------
+# random stream
+# ---------------------------------------------------------------------------
+
+def test_philox_matches_numpy(orc, usap):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        ctr = rng.integers(0, 2**63, 4, dtype=np.uint64)
+        key = rng.integers(0, 2**63, 2, dtype=np.uint64)
+        # numpy's Philox increments its counter before producing a block
+        prev = ctr.copy()
+        prev[0] = prev[0] - np.uint64(1) if prev[0] > 0 else prev[0]
+        if ctr[0] == 0:
+            continue
+        bg = np.random.Philox(key=key, counter=prev)
+        want = bg.random_raw(4).astype(np.uint64)
+        np.testing.assert_array_equal(orc.philox4x64(ctr, key), want)
+        np.testing.assert_array_equal(usap.philox4x64(ctr, key), want)
+
+
+def test_philox_known_answer(orc):
+    # Random123 philox4x64-10 known-answer vector for counter 0, key 0
+    out = orc.philox4x64([0, 0, 0, 0], [0, 0])
+    assert [hex(int(v)) for v in out] == [
+        "0x16554d9eca36314c", "0xdb20fe9d672d0fdc", "0xd7e772cee186176b", "0x7e68b68aec7ba23b"]
+
+
+def test_pick_enumerates_weights(orc):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        w = rng.integers(0, 5, rng.integers(1, 9)).astype(np.int64)
+        if w.sum() == 0:
+            w[0] = 1
+        counts = np.zeros(len(w), np.int64)
+        for t in range(int(w.sum())):
+            counts[orc.pick(w, t)] += 1
+        np.testing.assert_array_equal(counts, w)
+
+
+# ---------------------------------------------------------------------------
+# fixed-point logarithm and entropy
+# ---------------------------------------------------------------------------
+
+def test_lambda_table_accuracy(orc, usap):
+    getcontext().prec = 50
+    lam = orc.lambda_table(5000)
+    np.testing.assert_array_equal(lam, usap.lambda_table(5000))
+    assert lam[0] == 0 and lam[1] == 0
+    xs = list(range(1, 200)) + list(range(4000, 5001, 37))
+    for x in xs:
+        exact = Decimal(x).ln() * FRAC
+        assert abs(Decimal(int(lam[x])) - exact) <= Decimal(2) * (1 + exact / Decimal(2**52) * 4)
+
+
+def _data_S_textbook(M, dout, din):
+    s = 0.0
+    for a, b in zip(*np.nonzero(M)):
+        m = M[a, b]
+        s -= m * math.log(m / (dout[a] * din[b]))
+    return s
+
+
+def _model_S(N, E, B):
+    x = B * B / E
+    return E * (1 + x) * math.log(1 + x) - x * math.log(x) + N * math.log(B)
+
+
+@pytest.mark.parametrize("init", ["scc", "identity", "random"])
+def test_M_and_entropy(orc, usap, init):
+    g = random_graph(300, 2500, seed=2, weighted=True, self_loops=True)
+    o = orc.Oracle(g, threads=2, block_size=64)  # exercises the edge-list M path too
+    if init == "scc":
+        lab, B = o.scc()
+    elif init == "identity":
+        lab, B = np.arange(g.N, dtype=np.int32), g.N
+    else:
+        B = 17
+        lab = np.random.default_rng(3).integers(0, B, g.N).astype(np.int32)
+    o.set_labels(lab, B)
+    src, dst = g.edges()
+    M = np.zeros((B, B), np.int64)
+    np.add.at(M, (lab[src], lab[dst]), g.out_w.astype(np.int64))
+    np.testing.assert_array_equal(o.M_dense(), M)
+    do, di = o.degrees()
+    np.testing.assert_array_equal(do, M.sum(1))
+    np.testing.assert_array_equal(di, M.sum(0))
+    fx, S = o.entropy()
+    want = _data_S_textbook(M, M.sum(1), M.sum(0))
+    assert abs(fx / FRAC - want) <= 1e-9 * abs(want)
+    assert abs(S - (want + _model_S(g.N, g.E, B))) <= 1e-9 * abs(S)
+    assert abs(usap.model_S(g.N, g.E, B) - _model_S(g.N, g.E, B)) <= 1e-12 * abs(S)
+
+
+# ---------------------------------------------------------------------------
+# dS against brute force
+# ---------------------------------------------------------------------------
+
+def _fresh(orc, g, lab, B, **kw):
+    o = orc.Oracle(g, threads=1, **kw)
+    o.set_labels(lab, B)
+    return o
+
+
+@pytest.mark.parametrize("block_size", [1024, 8])
+def test_merge_dS_equals_entropy_difference(orc, block_size):
+    g = random_graph(120, 900, seed=4, weighted=True)
+    rng = np.random.default_rng(5)
+    B = 12
+    lab = rng.integers(0, B, g.N).astype(np.int32)
+    o = _fresh(orc, g, lab, B, block_size=block_size)
+    base, _ = o.entropy()
+    seen = 0
+    for r in range(B):
+        for idx in range(4):
+            s, d = o.merge_propose_one(r, idx, 9)
+            if s == r:
+                assert d == 0
+                continue
+            merged = np.where(lab == r, s, lab)
+            uniq, new = np.unique(merged, return_inverse=True)
+            o2 = _fresh(orc, g, new.astype(np.int32), len(uniq))
+            after, _ = o2.entropy()
+            # the merged partition has B-1 blocks only in the model term; the
+            # data term is what dS measures
+            assert d == after - base
+            seen += 1
+    assert seen > 20
+
+
+@pytest.mark.parametrize("block_size", [1024, 8])
+def test_nodal_dS_exact_equals_entropy_difference(orc, block_size):
+    g = random_graph(150, 1400, seed=6, weighted=True, self_loops=True)
+    rng = np.random.default_rng(7)
+    B = 9
+    lab = rng.integers(0, B, g.N).astype(np.int32)
+    o = _fresh(orc, g, lab, B, block_size=block_size, ds_mode=1)
+    base, _ = o.entropy()
+    checked = 0
+    for n in range(g.N):
+        for stream in range(3):
+            s, d, inf, acc = o.nodal_propose_one(n, stream)
+            r = lab[n]
+            if s == r:
+                assert d == 0 and acc == 0
+                continue
+            lab2 = lab.copy()
+            lab2[n] = s
+            after, _ = _fresh(orc, g, lab2, B).entropy()
+            assert inf == 0
+            assert d == after - base
+            assert acc == (d < 0)
+            checked += 1
+    assert checked > 200
+
+
+def _corner_T(M, r, s, dout, din, x_r, x_s, y_r, y_s):
+    """sum of the 4 corner T terms (double) for entries x and degrees y."""
+    tot = 0.0
+    for m, x, y in ((M[0], y_r, x_r), (M[1], y_s, x_r), (M[2], y_r, x_s), (M[3], y_s, x_s)):
+        if m:
+            if x * y == 0:
+                return math.inf
+            tot += m * math.log(m / (x * y))
+    return tot
+
+
+def test_nodal_dS_ref_mode_quirk(orc):
+    """ref mode (graph_parallel.hpp:1117-1145) differs from exact mode only in
+    the four (r,s) corner entries of the new rows: they miss the updates of the
+    in-edges from r and s.  Checked against an independent corner evaluation."""
+    g = random_graph(150, 1400, seed=8)
+    rng = np.random.default_rng(9)
+    B = 7
+    lab = rng.integers(0, B, g.N).astype(np.int32)
+    oref = _fresh(orc, g, lab, B, ds_mode=0)
+    oex = _fresh(orc, g, lab, B, ds_mode=1)
+    M = oref.M_dense()
+    dout, din = M.sum(1), M.sum(0)
+    src, dst = g.edges()
+    same, differ = 0, 0
+    for n in range(g.N):
+        s, dref, infr, _ = oref.nodal_propose_one(n, 4)
+        s2, dex, infx, _ = oex.nodal_propose_one(n, 4)
+        assert s == s2
+        r = lab[n]
+        if s == r:
+            continue
+        outs = dst[src == n]
+        ins = src[dst == n]
+        wo_r, wo_s = np.sum(lab[outs] == r), np.sum(lab[outs] == s)
+        wi_r, wi_s = np.sum(lab[ins] == r), np.sum(lab[ins] == s)
+        k_out, k_in = len(outs), len(ins)
+        dor2, dos2 = dout[r] - k_out, dout[s] + k_out
+        dir2, dis2 = din[r] - k_in, din[s] + k_in
+        ref_c = (M[r, r] - wo_r, M[r, s] - wo_s, M[s, r] + wo_r, M[s, s] + wo_s)
+        ex_c = (M[r, r] - wo_r - wi_r, M[r, s] - wo_s + wi_r, M[s, r] + wo_r - wi_s,
+                M[s, s] + wo_s + wi_s)
+        t_ref = _corner_T(ref_c, r, s, dout, din, dor2, dos2, dir2, dis2)
+        t_ex = _corner_T(ex_c, r, s, dout, din, dor2, dos2, dir2, dis2)
+        if math.isinf(t_ref):
+            assert infr == -1
+            differ += 1
+            continue
+        assert infr == 0 and infx == 0
+        want = dex / FRAC + (t_ex - t_ref)
+        assert abs(dref / FRAC - want) <= 1e-7 * (1 + abs(want))
+        if wi_r == 0 and wi_s == 0:
+            assert dref == dex
+            same += 1
+        else:
+            differ += 1
+    assert same > 10 and differ > 10
+
+
+def test_nodal_proposal_distribution(orc):
+    """Chi-square of the proposed block against the exact proposal law of
+    graph_parallel.hpp:1006-1063 for one node."""
+    g = random_graph(60, 500, seed=10)
+    B = 6
+    lab = (np.arange(g.N) % B).astype(np.int32)
+    o = _fresh(orc, g, lab, B)
+    M = o.M_dense()
+    da = M.sum(1) + M.sum(0)
+    n = int(np.argmax(np.diff(g.out_ptr) + np.diff(g.in_ptr)))
+    nb = np.concatenate([g.out_idx[g.out_ptr[n]:g.out_ptr[n + 1]],
+                         g.in_idx[g.in_ptr[n]:g.in_ptr[n + 1]]])
+    p = np.zeros(B)
+    for v in nb:
+        u = lab[v]
+        pu = B / (da[u] + B)
+        p += (pu / B + (1 - pu) * (M[u] + M[:, u]) / da[u]) / len(nb)
+    draws = 20000
+    counts = np.zeros(B)
+    for stream in range(draws):
+        counts[o.nodal_propose_one(n, stream)[0]] += 1
+    from scipy.stats import chisquare
+    assert chisquare(counts, p * draws).pvalue > 1e-4
+
+
+# ---------------------------------------------------------------------------
+# carry-out, SCC, golden section
+# ---------------------------------------------------------------------------
+
+def _greedy_merges(best, dS, B_to_merge, lab):
+    """Greedy merges in (dS, block) order as a union of classes."""
+    rep = list(range(len(best)))
+
+    def find(x):
+        while rep[x] != x:
+            x = rep[x]
+        return x
+
+    done = 0
+    for r in sorted(range(len(best)), key=lambda i: (dS[i], i)):
+        if done == B_to_merge:
+            break
+        t = find(best[r])
+        if t != r:
+            rep[r] = t
+            done += 1
+    roots = sorted({find(b) for b in set(lab.tolist())})
+    rank = {b: i for i, b in enumerate(roots)}
+    return np.array([rank[find(b)] for b in lab], np.int32)
+
+
+def test_carry_out(orc):
+    g = random_graph(200, 1500, seed=11)
+    rng = np.random.default_rng(12)
+    B = 40
+    lab = rng.integers(0, B, g.N).astype(np.int32)
+    o = _fresh(orc, g, lab, B)
+    best, dS = o.block_merge(3)
+    want = _greedy_merges(best, dS, 20, lab)
+    assert o.carry_out(20) == 0
+    np.testing.assert_array_equal(o.labels(), want)
+    assert o.B == 20
+
+
+def test_carry_out_infeasible(orc):
+    g = random_graph(50, 300, seed=13)
+    lab = (np.arange(g.N) % 10).astype(np.int32)
+    o = _fresh(orc, g, lab, 10)
+    o.block_merge(0)
+    assert o.carry_out(9) in (0, -1)
+    o2 = _fresh(orc, g, lab, 10)
+    o2.block_merge(0)
+    # at most B - (number of components of the best-merge graph) merges exist
+    assert o2.carry_out(10) == -1
+
+
+def _graph_from_edges(usap, edges, N):
+    s = np.array([e[0] for e in edges], np.int32)
+    d = np.array([e[1] for e in edges], np.int32)
+    return usap.Graph.from_edges(s, d, N)
+
+
+@pytest.mark.parametrize("case", ["cycle", "chain", "long_cycle", "two_sccs"])
+def test_scc_hand_cases(orc, usap, case):
+    if case == "cycle":
+        g, want = _graph_from_edges(usap, [(0, 1), (1, 2), (2, 0)], 3), [0, 0, 0]
+    elif case == "chain":
+        g, want = _graph_from_edges(usap, [(0, 1), (1, 2)], 3), [0, 1, 2]
+    elif case == "long_cycle":
+        # finish order is 49..0; the transpose DFS from 0 walks 0,49,48,..,31
+        # and stops at the depth cap (20 nodes); nodes 1..30 then each start a
+        # DFS whose only transpose neighbour is already labelled
+        g = _graph_from_edges(usap, [(i, (i + 1) % 50) for i in range(50)], 50)
+        want = None
+    else:  # {0,1} -> {2,3}
+        g, want = _graph_from_edges(usap, [(0, 1), (1, 0), (1, 2), (2, 3), (3, 2)], 4), [0, 0, 1, 1]
+    lab, B = orc.Oracle(g).scc()
+    lab2, B2 = usap.scc_init(g)
+    np.testing.assert_array_equal(lab, lab2)
+    assert B == B2
+    if want is not None:
+        np.testing.assert_array_equal(lab, want)
+    else:
+        assert B == 31 and sorted(np.bincount(lab).tolist()) == [1] * 30 + [20]
+        assert np.all(lab[[0] + list(range(31, 50))] == 0)
+
+
+def test_scc_random_graphs_match(orc, usap):
+    for seed in range(5):
+        g = random_graph(400, 1200, seed=20 + seed)
+        lab, B = orc.Oracle(g).scc()
+        lab2, B2 = usap.scc_init(g)
+        np.testing.assert_array_equal(lab, lab2)
+        assert B == B2 and lab.max() == B - 1
+
+
+INF = math.inf
+EMPTY = (0, INF, -1)
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_golden_section_script(orc, usap, impl):
+    step = orc.golden_update if impl == "oracle" else usap.golden_update
+    slots = [EMPTY, EMPTY, EMPTY]
+    rc, slots, nxt, btm = step(slots, (100, 50.0, 0))
+    assert (rc, nxt[2], btm) == (0, 0, 50) and slots[1][2] == 0
+    rc, slots, nxt, btm = step(slots, (50, 40.0, 1))
+    assert (rc, nxt[2], btm) == (0, 1, 25) and slots[0][2] == 0 and slots[1][2] == 1
+    rc, slots, nxt, btm = step(slots, (25, 45.0, 2))
+    # bracket [100, 50, 25]: larger side first, next B = 50 + round(50*0.618) = 81
+    assert (rc, nxt[2], btm) == (0, 0, 19) and [s[2] for s in slots] == [0, 1, 2]
+    rc, slots, nxt, btm = step(slots, (81, 42.0, 3))
+    # 81 worse than med (50) and above it -> large = 81; [81, 50, 25]: 31 >= 25
+    # -> next = 50 + round(31*0.618) = 69 from large (81)
+    assert (rc, nxt[2], btm) == (0, 3, 12) and [s[2] for s in slots] == [3, 1, 2]
+    rc, slots, nxt, btm = step(slots, (69, 39.0, 4))
+    # 69 better than med -> med.B (50) < 69 -> small = old med; med = 69:
+    # [81, 69, 50]; 12 < 19 -> next = 50 + round(19*0.618) = 62 from med
+    assert (rc, nxt[2], btm) == (0, 4, 7) and [s[2] for s in slots] == [3, 4, 1]
+    # bracket of width 2 ends the search at med
+    rc, slots, nxt, btm = step([(12, 5.0, 7), (11, 4.0, 8), (10, 6.0, 9)], (12, 4.5, 10))
+    assert rc == 1 and nxt[2] == 8
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_golden_section_invalid_bracket(orc, usap, impl):
+    step = orc.golden_update if impl == "oracle" else usap.golden_update
+    rc, slots, nxt, btm = step([EMPTY, EMPTY, EMPTY], (100, 50.0, 0))
+    rc, slots, nxt, btm = step(slots, (50, 60.0, 1))  # worse on the first halving
+    assert rc < 0
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_golden_section_stops_at_one_block(orc, usap, impl):
+    step = orc.golden_update if impl == "oracle" else usap.golden_update
+    rc, slots, nxt, btm = step([EMPTY, EMPTY, EMPTY], (1, 5.0, 0))
+    assert rc == 1 and btm == 0 and nxt[2] == 0
+
+
+# ---------------------------------------------------------------------------
+# end to end (oracle alone)
+# ---------------------------------------------------------------------------
+
+def test_oracle_partition_recovers_planted_c1(orc, usap):
+    g = usap.config_graph("c1")
+    o = orc.Oracle(g, threads=4, seed=1)
+    o.set_labels(*o.scc())
+    B, lab, st = o.partition()
+    assert st["error"] == 0
+    assert B == len(np.unique(g.truth))
+    assert usap.nmi(g.truth, lab) > 0.99
+
+
+def test_oracle_thread_count_invariance(orc, usap):
+    g = usap.config_graph("c1")
+    out = []
+    for t in (1, 3, 8):
+        o = orc.Oracle(g, threads=t, seed=5)
+        o.set_labels(*o.scc())
+        out.append(o.partition())
+    for B, lab, st in out[1:]:
+        assert B == out[0][0] and st == out[0][2]
+        np.testing.assert_array_equal(lab, out[0][1])

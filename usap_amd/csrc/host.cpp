@@ -265,11 +265,12 @@ int usap_generate_dcsbm(int32_t N, int32_t K, double alpha, int32_t min_block, i
   for (int32_t u = 0; u < N; ++u) {
     stamp[u] = u;
     for (int32_t j = 0; j < deg[u]; ++j) {
+      // target block once per edge; duplicates and self-loops redraw the node only
+      int32_t tb = lab[u];
+      if (K > 1 && se.uniform() >= p_in) tb = (int32_t)((lab[u] + 1 + se.index(K - 1)) % K);
+      const auto& m = members[tb];
       int32_t v = -1;
       for (int attempt = 0; attempt < 64 && v < 0; ++attempt) {
-        int32_t tb = lab[u];
-        if (K > 1 && se.uniform() >= p_in) tb = (int32_t)((lab[u] + 1 + se.index(K - 1)) % K);
-        const auto& m = members[tb];
         const int32_t c = m[se.index((int64_t)m.size())];
         if (stamp[c] != u) v = c;
       }
