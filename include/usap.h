@@ -164,6 +164,38 @@ int usap_restore(usap_engine* e, int32_t slot);
  * labels_out (nullable, [N]); stats (nullable). */
 int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats);
 
+/* Per-phase record of the last usap_partition (one per merge phase). */
+typedef struct usap_phase {
+  int32_t B;           /* blocks after the merge */
+  int32_t sweeps;      /* nodal sweeps run */
+  int64_t proposals;   /* nodal proposals evaluated */
+  int64_t moves;       /* accepted nodal moves */
+  double S;            /* MDL at the end of the phase */
+  double ms_merge;     /* merge proposals + carry-out */
+  double ms_nodal;     /* nodal sweeps */
+} usap_phase;
+
+/* Copies up to cap phase records of the last usap_partition; *n = count. */
+int usap_partition_log(usap_engine* e, int32_t cap, usap_phase* out, int32_t* n);
+
+/* Kernel profile: HIP-event time on the engine's stream, launches, and the
+ * algorithmic bytes the kernel moved (reads + writes the algorithm needs,
+ * each counted once; 0 where not tallied). */
+#define USAP_K_NODAL 0   /* nodal proposals */
+#define USAP_K_APPLY 1   /* accepted-move application */
+#define USAP_K_MERGE 2   /* merge proposals */
+#define USAP_K_ENTROPY 3 /* MDL reduction */
+#define USAP_K_BUILD 4   /* M rebuild (all its kernels) */
+#define USAP_NKERNELS 5
+typedef struct usap_kernel_prof {
+  int64_t launches;
+  double ms;
+  double bytes;
+} usap_kernel_prof;
+/* enable = 1: reset and start timing; 0: stop; -1: leave as is.  out
+ * (nullable, [USAP_NKERNELS]) receives the totals since the last reset. */
+int usap_profile(usap_engine* e, int32_t enable, usap_kernel_prof* out);
+
 /* Golden-section bracket update on (B, S, id) triples, exposed for tests.
  * ≙ _prepare_for_partition_next (graph_parallel.hpp:1322-1365).
  * slots_* [3] = large, med, small (B = 0, S = +inf, id = -1 when empty).

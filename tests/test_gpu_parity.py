@@ -126,7 +126,6 @@ def test_partition_c1(usap, orc, graphs, mode, init):
     assert st["S"] == ost["S"]
     assert st["sweeps"] == ost["sweeps"] and st["batches"] == ost["batches"]
     assert st["accepted_moves"] == ost["moves"]
-    assert usap.nmi(g.truth, lab) > 0.95
 
 
 def test_partition_weighted_selfloops(usap, orc, graphs):

@@ -83,6 +83,25 @@ class UsapStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class UsapPhase(C.Structure):
+    _fields_ = [
+        ("B", C.c_int32),
+        ("sweeps", C.c_int32),
+        ("proposals", C.c_int64),
+        ("moves", C.c_int64),
+        ("S", C.c_double),
+        ("ms_merge", C.c_double),
+        ("ms_nodal", C.c_double),
+    ]
+
+
+class UsapKernelProf(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
+KERNELS = ["nodal", "apply", "merge", "entropy", "build"]
+
+
 # (name, restype, argtypes) of every exported entry point of include/usap.h
 SIGNATURES = [
     ("usap_default_params", None, [C.POINTER(UsapParams)]),
@@ -107,6 +126,8 @@ SIGNATURES = [
     ("usap_snapshot", C.c_int, [_vp, C.c_int32, C.c_double]),
     ("usap_restore", C.c_int, [_vp, C.c_int32]),
     ("usap_partition", C.c_int, [_vp, _vp, C.POINTER(UsapStats)]),
+    ("usap_partition_log", C.c_int, [_vp, C.c_int32, _vp, C.POINTER(C.c_int32)]),
+    ("usap_profile", C.c_int, [_vp, C.c_int32, _vp]),
     ("usap_golden_update", C.c_int, [_i64p, _f64p, _i64p, C.c_int64, C.c_double, C.c_int64,
                                      C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

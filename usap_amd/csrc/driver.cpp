@@ -140,9 +140,14 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
   bool slot_used[4] = {false, false, false, false};
   uint64_t merge_stream = 0, nodal_stream = 0;
   double S = 0;
+  std::vector<usap_phase>& plog = usap::engine_phase_log(e);
+  plog.clear();
 
   for (int64_t phase = 0;; ++phase) {
     // prepare_merge + block_merge + perform_merge + prepare_move
+    usap_phase ph{};
+    const int64_t sweeps0 = st.sweeps, props0 = st.nodal_proposals, moves0 = st.accepted_moves;
+    const auto t_phase = clk::now();
     auto t0 = clk::now();
     if ((rc = usap_merge_propose(e, p.K, merge_stream++, nullptr, nullptr))) break;
     st.merge_proposals += (int64_t)B * p.K;
@@ -151,6 +156,9 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     if ((rc = usap_merge_apply(e, (int32_t)B_to_merge, &B))) break;
     st.ms_carry += ms_since(t0);
     st.phases++;
+    ph.B = B;
+    ph.ms_merge = ms_since(t_phase);
+    const auto t_nodal = clk::now();
     std::fill(itr_dS.begin(), itr_dS.end(), 0.0);
     for (int32_t itr = 0; itr < p.max_nodal_itr; ++itr) {
       // prepare_batch
@@ -194,13 +202,22 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
         }
       }
       st.ms_nodal += ms_since(t0);
+      if (p.verbose > 1)
+        std::fprintf(stderr, "usap:   sweep %d B=%d S=%.3f moves=%lld %.1f ms\n", itr, B, S,
+                     (long long)(st.accepted_moves - moves0), ms_since(t0));
       if (rc || converged) break;
     }
     if (rc) break;
+    ph.ms_nodal = ms_since(t_nodal);
     // prepare_for_next
     t0 = clk::now();
     if ((rc = usap_entropy(e, &S, nullptr, nullptr))) break;
     st.ms_entropy += ms_since(t0);
+    ph.sweeps = (int32_t)(st.sweeps - sweeps0);
+    ph.proposals = st.nodal_proposals - props0;
+    ph.moves = st.accepted_moves - moves0;
+    ph.S = S;
+    plog.push_back(ph);
     t0 = clk::now();
     int slot = -1;
     for (int i = 0; i < 4; ++i)

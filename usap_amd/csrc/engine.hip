@@ -66,6 +66,8 @@ enum : int32_t {
 struct Status {
   unsigned long long sum_lo;
   unsigned long long sum_hi;
+  unsigned long long nodal_bytes;  // algorithmic bytes of k_nodal
+  unsigned long long apply_bytes;  // algorithmic bytes of k_apply
   int32_t ninf;
   int32_t moves;
   int32_t flags;
@@ -237,11 +239,12 @@ __device__ __forceinline__ long count_le_lane(const MView& m, int64_t o, int32_t
 // running sum exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact
 // integer weights).  The caller guarantees t < total weight.
 __device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
-                                   int32_t excl) {
+                                   int32_t excl, long& bytes) {
   const int lane = lane_id();
   const int32_t B = M.B;
   const int64_t oM = M.off[u], oT = Mt.off[u];
   const int32_t cM = M.cap[u], cT = Mt.cap[u];
+  bytes += 24;  // off and cap of both rows
   if (cM == B && cT == B) {
     for (int32_t base = 0; base < B; base += kWave) {
       const int32_t i = base + lane;
@@ -249,6 +252,7 @@ __device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, i
       if (i < B && i != excl) w = (long)M.val[oM + i] + (long)Mt.val[oT + i];
       const long inc = wave_incl_scan(w);
       const long tot = __shfl(inc, kWave - 1);
+      bytes += 8 * (long)min(kWave, B - base);
       if (t < tot) {
         const unsigned long long hit = __ballot(inc > t);
         return base + (__ffsll((unsigned long long)hit) - 1);
@@ -257,6 +261,7 @@ __device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, i
     }
     return B - 1;
   }
+  bytes += 8 * ((long)(cM == B ? B : cM) + (long)(cT == B ? B : cT));
   int32_t lo = 0, hi = B - 1;
   while (lo < hi) {
     const int32_t mid = lo + (hi - lo) / 2;
@@ -460,14 +465,15 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t& s
   if (k == 0) {
     s = (int32_t)usap::idx_of(x[2], B);
   } else {
-    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1);
+    long bytes = 0;
+    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1, bytes);
     const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
     if (usap::uniform_branch(x[1], da, B)) {
       s = usap::uniform_not_r(x[2], r, B);
     } else {
       const int64_t W2 = da - m_get(a.M, u, r) - m_get(a.Mt, u, r);
       s = W2 == 0 ? usap::uniform_not_r(x[2], r, B)
-                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r);
+                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r, bytes);
     }
   }
   s_out = s;
@@ -586,7 +592,7 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
                                             int32_t cap, const NodalArgs& a, int32_t n,
                                             int32_t r, int32_t s, int64_t o0, int64_t kco,
                                             int64_t i0, int64_t kci, int64_t k_out,
-                                            int64_t k_in, bool& inf) {
+                                            int64_t k_in, bool& inf, long& bytes) {
   const int lane = lane_id();
   const uint32_t mask = (uint32_t)cap - 1;
   const int64_t* lam = a.lam;
@@ -612,7 +618,7 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
   wave_fence();
   // changed entries of rows r, s (out-edges) and columns r, s (in-edges)
   i128 acc = 0;
-  long Wo = 0, Wi = 0, wor = 0, wos = 0, wir = 0, wis = 0;
+  long Wo = 0, Wi = 0, wor = 0, wos = 0, wir = 0, wis = 0, lookups = 0;
   bool neg = false;
   for (int32_t i = lane; i < cap; i += kWave) {
     const int32_t b = tk[i];
@@ -630,12 +636,14 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
     }
     if (wo > 0) {
       const int64_t Rr = m_get(a.M, r, b), Rs = m_get(a.M, s, b);
+      lookups += 2;
       neg |= Rr - wo < 0;
       acc += Lterm(lam, Rr) + Lterm(lam, Rs) - Lterm(lam, Rr - wo) - Lterm(lam, Rs + wo);
       Wo += wo;
     }
     if (wi > 0) {
       const int64_t Cr = m_get(a.Mt, r, b), Cs = m_get(a.Mt, s, b);
+      lookups += 2;
       neg |= Cr - wi < 0;
       acc += Lterm(lam, Cr) + Lterm(lam, Cs) - Lterm(lam, Cr - wi) - Lterm(lam, Cs + wi);
       Wi += wi;
@@ -650,6 +658,9 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
   wir = wave_sum(wir);
   wis = wave_sum(wis);
   wself = wave_sum(wself);
+  // neighbour ids + labels (+ weights), 2 lookups (key + count) per changed
+  // entry, 4 corner lookups, 4 degrees
+  bytes += (a.g.out_w ? 12 : 8) * (kco + kci) + 8 * (wave_sum(lookups) + 4) + 16;
   // corner (rows r, s at columns r, s) and the degree factors
   const int64_t Mrr = m_get(a.M, r, r), Mrs = m_get(a.M, r, s);
   const int64_t Msr = m_get(a.M, s, r), Mss = m_get(a.M, s, s);
@@ -705,6 +716,8 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     k_in = wave_sum(si);
   }
   const int64_t k = k_out + k_in;
+  // 4 CSR/CSC offsets, own label, outputs (s, dS, inf, accept)
+  long bytes = 32 + 4 + 28 + (a.g.out_w ? 4 * (kco + kci) : 0);
   uint64_t x[4];
   usap::philox4x64((uint64_t)n, 0, a.stream, usap::kTagNodal, a.seed, 0, x);
   int32_t s;
@@ -714,10 +727,11 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     const int32_t v = pick_neighbor(a.g, o0, kco, i0, kci, usap::idx_of(x[0], k));
     const int32_t u = a.labels[v];
     const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
+    bytes += 16;  // neighbour id, its label, d_out[u], d_in[u]
     if (usap::uniform_branch(x[1], da, B))
       s = (int32_t)usap::idx_of(x[2], B);
     else
-      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1);
+      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1, bytes);
   }
   i128 dS = 0;
   bool inf = false;
@@ -729,10 +743,10 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
       while (cap < 2 * (kco + kci)) cap <<= 1;
       int32_t* base = a.big_tab + bo;
       dS = nodal_delta(base, base + cap, base + 2 * cap, cap, a, n, r, s, o0, kco, i0, kci,
-                       k_out, k_in, inf);
+                       k_out, k_in, inf, bytes);
     } else {
       dS = nodal_delta(s_tab[w][0], s_tab[w][1], s_tab[w][2], kTab, a, n, r, s, o0, kco, i0,
-                       kci, k_out, k_in, inf);
+                       kci, k_out, k_in, inf, bytes);
     }
     acc = (inf || dS < 0) ? 1 : 0;
   }
@@ -742,6 +756,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     a.prop_lo[item] = (unsigned long)(u128)dS;
     a.prop_inf[item] = inf ? -1 : 0;
     a.prop_acc[item] = acc;
+    atomicAdd(&a.status->nodal_bytes, (unsigned long long)bytes);
   }
 }
 
@@ -805,6 +820,8 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   wo_sum = wave_sum(wo_sum);
   wi_sum = wave_sum(wi_sum);
   if (lane == 0) {
+    atomicAdd(&a.status->apply_bytes,
+              (unsigned long long)(44 + 60 * ((o1 - o0) + (i1 - i0)) + 32 + 32));
     if (wo_sum) {
       atomicSub(&a.d_out[r], (int32_t)wo_sum);
       atomicAdd(&a.d_out[s], (int32_t)wo_sum);
@@ -837,6 +854,8 @@ __global__ void k_labels(int32_t beg, int32_t end, int32_t* labels, const int32_
   if (i == 0) {
     next_status->sum_lo = 0;
     next_status->sum_hi = 0;
+    next_status->nodal_bytes = 0;
+    next_status->apply_bytes = 0;
     next_status->ninf = 0;
     next_status->moves = 0;
     next_status->flags = 0;
@@ -915,6 +934,11 @@ struct usap_engine {
   int32_t snap_B[kNumSnap] = {};
   double snap_S[kNumSnap] = {};
   int32_t rebuilds = 0;
+  // profiling (HIP events on the engine's stream)
+  bool prof = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  usap_kernel_prof kp[USAP_NKERNELS] = {};
+  std::vector<usap_phase> phase_log;
 };
 
 namespace {
@@ -985,10 +1009,29 @@ double fx_to_double(i128 v) {
 
 i128 mk128(long hi, unsigned long lo) { return (i128)(((u128)(unsigned long)hi << 64) | (u128)lo); }
 
+// elapsed ms between two recorded events (0 when profiling is off)
+float ev_ms(usap_engine* e, int a, int b) {
+  float ms = 0.f;
+  if (e->prof && hipEventElapsedTime(&ms, e->ev[a], e->ev[b]) != hipSuccess) ms = 0.f;
+  return ms;
+}
+
+void ev_rec(usap_engine* e, int i) {
+  if (e->prof) (void)hipEventRecord(e->ev[i], e->stream);
+}
+
+void prof_add(usap_engine* e, int k, double ms, int64_t launches, double bytes) {
+  if (!e->prof) return;
+  e->kp[k].ms += ms;
+  e->kp[k].launches += launches;
+  e->kp[k].bytes += bytes;
+}
+
 // Full M build from the labels (graph_parallel.hpp:703-759).
 int rebuild_M(usap_engine* e) {
   const int32_t B = e->B;
   hipStream_t st = e->stream;
+  ev_rec(e, 0);
   HIPCHK(hipMemsetAsync(e->d_out, 0, sizeof(int32_t) * B, st));
   HIPCHK(hipMemsetAsync(e->d_in, 0, sizeof(int32_t) * B, st));
   HIPCHK(hipMemsetAsync(e->ecnt_out, 0, sizeof(int32_t) * B, st));
@@ -1040,9 +1083,11 @@ int rebuild_M(usap_engine* e) {
         e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), view(e->Mt, B),
         e->build_flags);
   HIPCHK(hipGetLastError());
+  ev_rec(e, 1);
   int32_t flags = 0;
   HIPCHK(hipMemcpyAsync(&flags, e->build_flags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  prof_add(e, USAP_K_BUILD, ev_ms(e, 0, 1), 1, 0.0);
   if (flags & (kFlagFull | kFlagMissing))
     return set_err(e, USAP_ESTATE, "M build overflowed a row slab");
   e->rebuilds++;
@@ -1058,6 +1103,7 @@ int check_engine(usap_engine* e) {
 }  // namespace
 
 namespace usap {
+std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log; }
 const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
 int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
@@ -1129,6 +1175,8 @@ int usap_destroy(usap_engine* e) {
   dfree(e->ent_part);
   if (e->h_ent_part) (void)hipHostFree(e->h_ent_part);
   for (int i = 0; i < kNumSnap; ++i) dfree(e->snap_labels[i]);
+  for (hipEvent_t& ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return USAP_OK;
@@ -1296,6 +1344,7 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     TRYH(hipMemcpyAsync(e->lam, lam.data(), sizeof(int64_t) * (wtot + 1), hipMemcpyHostToDevice, st));
     TRYH(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, st));
     TRYH(hipMemsetAsync(e->status, 0, 2 * sizeof(Status), st));
+    for (hipEvent_t& ev : e->ev) TRYH(hipEventCreate(&ev));
     TRYH(hipStreamSynchronize(st));
   }
 #undef TRY
@@ -1385,12 +1434,16 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
 int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
   if (int rc = check_engine(e)) return rc;
   hipStream_t st = e->stream;
+  ev_rec(e, 0);
   k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.val, e->M.total, e->d_out, e->d_in, e->B, e->lam,
                                            e->ent_part);
   HIPCHK(hipGetLastError());
+  ev_rec(e, 1);
   HIPCHK(hipMemcpyAsync(e->h_ent_part, e->ent_part, sizeof(long) * 2 * kEntropyBlocks,
                         hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  prof_add(e, USAP_K_ENTROPY, ev_ms(e, 0, 1), 1,
+           4.0 * (double)e->M.total + 8.0 * e->B + 16.0 * kEntropyBlocks);
   i128 t = 0;
   for (int i = 0; i < kEntropyBlocks; ++i)
     t += mk128(e->h_ent_part[2 * i], (unsigned long)e->h_ent_part[2 * i + 1]);
@@ -1420,8 +1473,10 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   a.best_hi = e->best_hi;
   a.best_lo = e->best_lo;
   const int nw = std::min(K, kMaxMergeWaves);
+  ev_rec(e, 0);
   k_merge<<<B, nw * kWave, 0, e->stream>>>(a);
   HIPCHK(hipGetLastError());
+  ev_rec(e, 1);
   e->merge_B = B;
   if (best_s || best_dS) {
     std::vector<long> h((size_t)B);
@@ -1438,6 +1493,7 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   } else {
     HIPCHK(hipStreamSynchronize(e->stream));
   }
+  prof_add(e, USAP_K_MERGE, ev_ms(e, 0, 1), 1, 0.0);
   return USAP_OK;
 }
 
@@ -1556,7 +1612,9 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
   Status* nxt = e->status + (e->status_cur ^ 1);
   a.status = cur;
   const unsigned grid = blocks_for(n, kNodalWaves);
+  ev_rec(e, 0);
   k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
+  ev_rec(e, 1);
   ApplyArgs b;
   b.g = a.g;
   b.M = a.M;
@@ -1576,10 +1634,13 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
   k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
   k_labels<<<blocks_for(n, 256), 256, 0, st>>>(begin, end, e->labels, e->prop_s, e->prop_acc, nxt);
   HIPCHK(hipGetLastError());
+  ev_rec(e, 2);
   HIPCHK(hipMemcpyAsync(e->h_status, cur, sizeof(Status), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   e->status_cur ^= 1;
   const Status& s = *e->h_status;
+  prof_add(e, USAP_K_NODAL, ev_ms(e, 0, 1), 1, (double)s.nodal_bytes);
+  prof_add(e, USAP_K_APPLY, ev_ms(e, 1, 2), 2, (double)s.apply_bytes + 4.0 * n);
   if (s.flags & (kFlagMissing | kFlagNegative))
     return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
   if (s.flags & (kFlagLoad | kFlagFull)) {
@@ -1617,6 +1678,27 @@ int usap_nodal_detail(usap_engine* e, int32_t* s, int64_t* hi, uint64_t* lo, int
     if (accept) HIPCHK(hipMemcpyAsync(accept, e->prop_acc, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
   }
+  return USAP_OK;
+}
+
+int usap_profile(usap_engine* e, int32_t enable, usap_kernel_prof* out) {
+  if (!e) return USAP_EINVAL;
+  if (out)
+    for (int k = 0; k < USAP_NKERNELS; ++k) out[k] = e->kp[k];
+  if (enable == 1) {
+    for (int k = 0; k < USAP_NKERNELS; ++k) e->kp[k] = usap_kernel_prof{};
+    e->prof = true;
+  } else if (enable == 0) {
+    e->prof = false;
+  }
+  return USAP_OK;
+}
+
+int usap_partition_log(usap_engine* e, int32_t cap, usap_phase* out, int32_t* n) {
+  if (!e || !n) return USAP_EINVAL;
+  const int32_t m = (int32_t)e->phase_log.size();
+  for (int32_t i = 0; i < m && i < cap && out; ++i) out[i] = e->phase_log[i];
+  *n = m;
   return USAP_OK;
 }
 

@@ -1,11 +1,15 @@
 // engine_internal.hpp -- engine accessors used by the host driver inside the
 // library (not part of the public C-ABI).
 #pragma once
+#include <vector>
+
 #include "usap.h"
 
 namespace usap {
 // Host copy of the graph the engine was created with.
 const usap_graph* engine_host_graph(const usap_engine* e);
+// Per-phase records of the running / last partition (usap_partition_log).
+std::vector<usap_phase>& engine_phase_log(usap_engine* e);
 const usap_params* engine_params(const usap_engine* e);
 int32_t engine_rebuilds(const usap_engine* e);
 void engine_set_error(usap_engine* e, const char* msg);

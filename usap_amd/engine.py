@@ -3,7 +3,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import UsapParams, UsapStats, check, lib, ptr
+from ._lib import KERNELS, UsapKernelProf, UsapParams, UsapPhase, UsapStats, check, lib, ptr
 
 FRAC = 48
 INIT_SCC, INIT_IDENTITY = 0, 1
@@ -141,6 +141,21 @@ class Engine:
         st = UsapStats()
         self._ck(lib().usap_partition(self.h, lab.ctypes.data, C.byref(st)))
         return lab, st.as_dict()
+
+    def partition_log(self):
+        n = C.c_int32()
+        self._ck(lib().usap_partition_log(self.h, 0, None, C.byref(n)))
+        arr = (UsapPhase * max(n.value, 1))()
+        self._ck(lib().usap_partition_log(self.h, n.value, arr, C.byref(n)))
+        return [{k: getattr(a, k) for k, _ in UsapPhase._fields_} for a in arr[:n.value]]
+
+    def profile(self, enable=-1):
+        """enable=1 resets and starts kernel timing, 0 stops, -1 queries.
+        Returns {kernel: {launches, ms, bytes}} accumulated so far."""
+        arr = (UsapKernelProf * len(KERNELS))()
+        self._ck(lib().usap_profile(self.h, enable, arr))
+        return {k: {"launches": a.launches, "ms": a.ms, "bytes": a.bytes}
+                for k, a in zip(KERNELS, arr)}
 
 
 def scc_init(graph, depth=20):
