@@ -199,8 +199,9 @@ int usap_profile(usap_engine* e, int32_t enable, usap_kernel_prof* out);
 /* Golden-section bracket update on (B, S, id) triples, exposed for tests.
  * ≙ _prepare_for_partition_next (graph_parallel.hpp:1322-1365).
  * slots_* [3] = large, med, small (B = 0, S = +inf, id = -1 when empty).
- * Returns 1 if the optimal B is found, 0 to continue, USAP_ESTATE if the
- * bracket is invalid. */
+ * Returns 1 if the optimal B is found (also when the entropy rose on the
+ * second halving, where the reference is undefined: the search stops at med),
+ * 0 to continue. */
 int usap_golden_update(int64_t* slot_B, double* slot_S, int64_t* slot_id, int64_t cur_B,
                        double cur_S, int64_t cur_id, double rate, int64_t* next_B,
                        double* next_S, int64_t* next_id, int64_t* B_to_merge);

@@ -965,8 +965,7 @@ int orc_golden_update(orc_snap slots[3], orc_snap cur, double rate, orc_snap* ne
     *B_to_merge = (int64_t)floor((double)med->B * rate);
     return *B_to_merge == 0 ? 1 : 0;
   }
-  if (large->id < 0) return -1;
-  if (large->B - small->B == 2) {
+  if (large->id < 0 || large->B - small->B == 2) { /* large unset: stop at med */
     *next = *med;
     return 1;
   }
@@ -1067,10 +1066,6 @@ int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats) {
     orc_snap cur = {st->B, S, phase};
     orc_snap next;
     int rc = orc_golden_update(slots, cur, p->reduction_rate, &next, &Btm);
-    if (rc < 0) {
-      err = -4;
-      break;
-    }
     if (next.id != phase) orc_set_labels(st, saved[next.id], (int32_t)next.B);
     for (int64_t i = 0; i <= phase; ++i) {
       if (!saved[i]) continue;

@@ -148,8 +148,9 @@ typedef struct {
  * slots[0..2] = {large, med, small}; cur = the partition just finished.
  * Updates the slots, writes the partition to continue from (*next) and its
  * B_to_merge.  Returns 1 if the optimal B is found (*next is the answer),
- * 0 to continue, -1 if the bracket is invalid (large never set while small
- * is: the reference's size_t arithmetic is undefined there). */
+ * 0 to continue.  When small is set while large never was (the entropy rose
+ * on the second halving) the reference's size_t arithmetic is undefined; the
+ * search then stops at med (returns 1). */
 int orc_golden_update(orc_snap slots[3], orc_snap cur, double rate, orc_snap* next,
                       int64_t* B_to_merge);
 

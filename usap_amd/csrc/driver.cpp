@@ -37,9 +37,10 @@ double fx_to_double(i128 v) {
 }
 
 // _prepare_for_partition_next, graph_parallel.hpp:1322-1365.  Returns 1 when
-// the optimal B is found, 0 to continue, USAP_ESTATE when the bracket is
-// invalid (large never set while small is: the reference's size_t
-// arithmetic is undefined there).
+// the optimal B is found, 0 to continue.  When the entropy already rises on
+// the second halving, small is set while large never was; the reference's
+// size_t arithmetic is undefined there (it restores an empty partition), so
+// the search stops and returns med, the best partition found.
 int golden(Snap& large, Snap& med, Snap& small, const Snap& cur, double rate, Snap& next,
            int64_t& B_to_merge) {
   if (cur.S <= med.S) {
@@ -60,8 +61,7 @@ int golden(Snap& large, Snap& med, Snap& small, const Snap& cur, double rate, Sn
     B_to_merge = (int64_t)std::floor((double)med.B * rate);
     return B_to_merge == 0 ? 1 : 0;
   }
-  if (large.id < 0) return USAP_ESTATE;
-  if (large.B - small.B == 2) {
+  if (large.id < 0 || large.B - small.B == 2) {
     next = med;
     return 1;
   }
@@ -234,11 +234,6 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     slot_of.push_back(slot);
     Snap cur{B, S, phase}, next;
     const int gr = golden(large, med, small, cur, p.reduction_rate, next, B_to_merge);
-    if (gr < 0) {
-      rc = gr;
-      usap::engine_set_error(e, "golden-section bracket invalid (large never set)");
-      break;
-    }
     if (next.id != phase) {
       if ((rc = usap_restore(e, slot_of[next.id]))) break;
       B = (int32_t)next.B;
