@@ -173,6 +173,11 @@ typedef struct usap_phase {
   double S;            /* MDL at the end of the phase */
   double ms_merge;     /* merge proposals + carry-out */
   double ms_nodal;     /* nodal sweeps */
+  double k_nodal_ms;   /* kernel times in this phase (profiling on, else 0) */
+  double k_apply_ms;
+  double k_merge_ms;
+  double k_build_ms;
+  int32_t rebuilds;    /* full M rebuilds in this phase */
 } usap_phase;
 
 /* Copies up to cap phase records of the last usap_partition; *n = count. */

@@ -92,6 +92,11 @@ class UsapPhase(C.Structure):
         ("S", C.c_double),
         ("ms_merge", C.c_double),
         ("ms_nodal", C.c_double),
+        ("k_nodal_ms", C.c_double),
+        ("k_apply_ms", C.c_double),
+        ("k_merge_ms", C.c_double),
+        ("k_build_ms", C.c_double),
+        ("rebuilds", C.c_int32),
     ]
 
 

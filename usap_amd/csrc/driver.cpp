@@ -147,6 +147,9 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     // prepare_merge + block_merge + perform_merge + prepare_move
     usap_phase ph{};
     const int64_t sweeps0 = st.sweeps, props0 = st.nodal_proposals, moves0 = st.accepted_moves;
+    usap_kernel_prof kp0[USAP_NKERNELS];
+    usap_profile(e, -1, kp0);
+    const int32_t rb0 = usap::engine_rebuilds(e);
     const auto t_phase = clk::now();
     auto t0 = clk::now();
     if ((rc = usap_merge_propose(e, p.K, merge_stream++, nullptr, nullptr))) break;
@@ -217,6 +220,13 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     ph.proposals = st.nodal_proposals - props0;
     ph.moves = st.accepted_moves - moves0;
     ph.S = S;
+    usap_kernel_prof kp1[USAP_NKERNELS];
+    usap_profile(e, -1, kp1);
+    ph.k_nodal_ms = kp1[USAP_K_NODAL].ms - kp0[USAP_K_NODAL].ms;
+    ph.k_apply_ms = kp1[USAP_K_APPLY].ms - kp0[USAP_K_APPLY].ms;
+    ph.k_merge_ms = kp1[USAP_K_MERGE].ms - kp0[USAP_K_MERGE].ms;
+    ph.k_build_ms = kp1[USAP_K_BUILD].ms - kp0[USAP_K_BUILD].ms;
+    ph.rebuilds = usap::engine_rebuilds(e) - rb0;
     plog.push_back(ph);
     t0 = clk::now();
     int slot = -1;
@@ -247,9 +257,12 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     }
     st.ms_golden += ms_since(t0);
     if (p.verbose)
-      std::fprintf(stderr, "usap: phase %lld B=%d S=%.6f bracket=[%lld,%lld,%lld] next B=%lld\n",
-                   (long long)phase, B, S, (long long)large.B, (long long)med.B,
-                   (long long)small.B, (long long)(next.B - B_to_merge));
+      std::fprintf(stderr,
+                   "usap: phase %lld B=%d S=%.6f bracket=[%lld,%lld,%lld] next B=%lld sweeps=%d "
+                   "ms merge=%.1f nodal=%.1f k_nodal=%.1f k_apply=%.1f build=%.1f rebuilds=%d\n",
+                   (long long)phase, ph.B, ph.S, (long long)large.B, (long long)med.B,
+                   (long long)small.B, (long long)(next.B - B_to_merge), ph.sweeps, ph.ms_merge,
+                   ph.ms_nodal, ph.k_nodal_ms, ph.k_apply_ms, ph.k_build_ms, ph.rebuilds);
     if (gr == 1) {
       S = next.S;
       break;

@@ -326,10 +326,13 @@ __global__ void k_degrees(GraphView g, const int32_t* __restrict__ labels, int32
   atomicAdd(&bsize[a], 1);
 }
 
+// Slab capacity of a row with ecnt edges: a power of two >= 4*min(B, ecnt)+16
+// (so keys inserted by later moves fit without a rebuild), or a direct row of
+// B slots when that is no larger.
 __device__ __forceinline__ int32_t cap_for(int32_t B, int32_t ecnt) {
   const int64_t bound = ecnt < B ? ecnt : B;
-  int64_t c = 16;
-  while (c < 2 * bound + 8) c <<= 1;
+  int64_t c = 32;
+  while (c < 4 * bound + 16) c <<= 1;
   return c >= B ? B : (int32_t)c;
 }
 
@@ -745,7 +748,9 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
       dS = nodal_delta(base, base + cap, base + 2 * cap, cap, a, n, r, s, o0, kco, i0, kci,
                        k_out, k_in, inf, bytes);
     } else {
-      dS = nodal_delta(s_tab[w][0], s_tab[w][1], s_tab[w][2], kTab, a, n, r, s, o0, kco, i0,
+      int32_t cap = 64;  // >= 4/3 of the distinct neighbour blocks, <= kTab
+      while (cap < 2 * (kco + kci) && cap < kTab) cap <<= 1;
+      dS = nodal_delta(s_tab[w][0], s_tab[w][1], s_tab[w][2], cap, a, n, r, s, o0, kco, i0,
                        kci, k_out, k_in, inf, bytes);
     }
     acc = (inf || dS < 0) ? 1 : 0;
