@@ -73,6 +73,7 @@ typedef struct usap_params {
   uint64_t seed;                   /* random-stream key */
   int32_t device;                  /* HIP device ordinal */
   int32_t verbose;                 /* print per-phase progress to stderr */
+  int32_t slab_slack;              /* hashed M row capacity = pow2 >= slack*edges+16 (4; 1..16) */
 } usap_params;
 
 /* Partition statistics (usap_partition).  Times are host wall-clock ms. */
@@ -145,6 +146,28 @@ int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new);
  * dS_sum = sum of accepted dS (may be -inf in ref mode). */
 int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream,
                      double* dS_sum, int32_t* n_moves);
+/* One nodal sweep: batches [0,b), [b,2b), ... of size `batch`, the i-th with
+ * random stream stream0 + i, each followed on the device by the reference's
+ * update step (graph_parallel.hpp:433-458): the sweep's cumulative accepted
+ * dS (dS_sum), S += dS_sum, and, when `check` is set (itr >= window-1), the
+ * convergence test -mean < thr * S, which ends the sweep.  Equal to calling
+ * usap_nodal_batch batch by batch with that test on the host; one host
+ * synchronisation per sweep. */
+typedef struct usap_sweep {
+  int32_t batches;     /* batches executed */
+  int32_t converged;   /* the convergence test fired after the last one */
+  int64_t proposals;   /* nodal proposals evaluated */
+  int64_t moves;       /* accepted moves */
+  double S;            /* S after the last executed batch */
+  double dS_sum;       /* cumulative accepted dS of the sweep (itr_delta_entropy[itr]) */
+  int64_t dS_hi;       /* the same, exact int128 words */
+  uint64_t dS_lo;
+  int32_t dS_inf;      /* -1 when an accepted dS was -inf */
+  int32_t rebuilds;    /* full M rebuilds after slab overflows */
+} usap_sweep;
+int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, double mean,
+                     double thr, int32_t check, usap_sweep* out);
+
 /* Exact sum of the last batch: int128 words and inf flag (0 or -1). */
 int usap_nodal_sum_fx(usap_engine* e, int64_t* hi, uint64_t* lo, int32_t* inf);
 /* Per-node results of the last batch ([end-begin] each, nullable):

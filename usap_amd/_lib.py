@@ -56,6 +56,7 @@ class UsapParams(C.Structure):
         ("seed", C.c_uint64),
         ("device", C.c_int32),
         ("verbose", C.c_int32),
+        ("slab_slack", C.c_int32),
     ]
 
 
@@ -100,6 +101,24 @@ class UsapPhase(C.Structure):
     ]
 
 
+class UsapSweep(C.Structure):
+    _fields_ = [
+        ("batches", C.c_int32),
+        ("converged", C.c_int32),
+        ("proposals", C.c_int64),
+        ("moves", C.c_int64),
+        ("S", C.c_double),
+        ("dS_sum", C.c_double),
+        ("dS_hi", C.c_int64),
+        ("dS_lo", C.c_uint64),
+        ("dS_inf", C.c_int32),
+        ("rebuilds", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class UsapKernelProf(C.Structure):
     _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
 
@@ -125,6 +144,8 @@ SIGNATURES = [
     ("usap_merge_apply", C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),
     ("usap_nodal_batch", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_uint64,
                                    C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("usap_nodal_sweep", C.c_int, [_vp, C.c_int32, C.c_uint64, C.c_double, C.c_double,
+                                   C.c_double, C.c_int32, C.POINTER(UsapSweep)]),
     ("usap_nodal_sum_fx", C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_int32)]),
     ("usap_nodal_detail", C.c_int, [_vp, _i32p, _i64p, _u64p, _i32p, _i32p]),

@@ -1,7 +1,7 @@
 // driver.cpp -- host C++ restatement of Graph_P::partition()
 // (graph_parallel.hpp:322-512) and its golden-section search over B
 // (graph_parallel.hpp:1322-1365), running the hot path through the engine's
-// C-ABI (usap_merge_propose / usap_merge_apply / usap_nodal_batch /
+// C-ABI (usap_merge_propose / usap_merge_apply / usap_nodal_sweep /
 // usap_entropy / usap_snapshot / usap_restore).
 //
 // Random streams: merge phase p draws with stream p, the j-th nodal batch of
@@ -18,23 +18,11 @@
 
 namespace {
 
-using usap::i128;
-using usap::u128;
-
 struct Snap {
   int64_t B = 0;
   double S = INFINITY;
   int64_t id = -1;
 };
-
-double fx_to_double(i128 v) {
-  const bool neg = v < 0;
-  const u128 m = neg ? (u128)0 - (u128)v : (u128)v;
-  const uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
-  const double d = std::ldexp((double)hi, 64 - usap::kLambdaFrac) +
-                   std::ldexp((double)lo, -usap::kLambdaFrac);
-  return neg ? -d : d;
-}
 
 // _prepare_for_partition_next, graph_parallel.hpp:1322-1365.  Returns 1 when
 // the optimal B is found, 0 to continue.  When the entropy already rises on
@@ -174,36 +162,21 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
         mean /= (double)(W - 1);
       }
       const bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
-      i128 cum = 0;
-      bool cum_inf = false, converged = false;
       st.sweeps++;
       t0 = clk::now();
-      for (int32_t beg = 0; beg < N; beg += batch) {
-        const int32_t end = beg + batch > N ? N : beg + batch;
-        int32_t moves = 0;
-        if ((rc = usap_nodal_batch(e, beg, end, nodal_stream++, nullptr, &moves))) break;
-        int64_t hi;
-        uint64_t lo;
-        int32_t inf;
-        usap_nodal_sum_fx(e, &hi, &lo, &inf);
-        st.batches++;
-        st.nodal_proposals += end - beg;
-        st.accepted_moves += moves;
-        if (inf)
-          cum_inf = true;
-        else
-          cum += (i128)(((u128)(uint64_t)hi << 64) | (u128)lo);
-        const double dS_sum = cum_inf ? -INFINITY : fx_to_double(cum);
-        itr_dS[itr] = dS_sum;
-        S += dS_sum;
-        if (itr >= W - 1) {
-          const double thr = isf ? p.thr2 : p.thr1;
-          if (-mean < thr * S) {
-            converged = true;
-            break;
-          }
-        }
-      }
+      // fetch_batch / nodal_move / update / check_batch (graph_parallel.hpp:397-468),
+      // the update step and its convergence test run on the device
+      usap_sweep sw{};
+      if ((rc = usap_nodal_sweep(e, batch, nodal_stream, S, mean, isf ? p.thr2 : p.thr1,
+                                 itr >= W - 1, &sw)))
+        break;
+      nodal_stream += (uint64_t)sw.batches;
+      st.batches += sw.batches;
+      st.nodal_proposals += sw.proposals;
+      st.accepted_moves += sw.moves;
+      itr_dS[itr] = sw.dS_sum;
+      S = sw.S;
+      const bool converged = sw.converged != 0;
       st.ms_nodal += ms_since(t0);
       if (p.verbose > 1)
         std::fprintf(stderr, "usap:   sweep %d B=%d S=%.3f moves=%lld %.1f ms\n", itr, B, S,

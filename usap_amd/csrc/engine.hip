@@ -20,8 +20,10 @@
 //        one wavefront per proposal
 //   k_nodal   : one nodal proposal per node against the frozen M
 //        (graph_parallel.hpp:990-1222), one wavefront per node
-//   k_apply / k_labels : accepted moves -> M, degrees (int32 atomics) and
-//        labels; equals the reference's full rebuild (graph_parallel.hpp:438-443)
+//   k_apply   : accepted moves -> M, degrees (int32 atomics) and labels;
+//        equals the reference's full rebuild (graph_parallel.hpp:438-443); its
+//        last block runs the sweep's update step and convergence test
+//        (graph_parallel.hpp:433-458) so a whole sweep runs without the host
 //   k_entropy : data_S over all non-zeros (graph_parallel.hpp:1281-1319)
 //
 // dS uses the factorised form of the reference's term sums: terms whose count
@@ -52,7 +54,11 @@ constexpr int kWave = 64;
 constexpr int kNodalWaves = 4;           // wavefronts per workgroup in k_nodal / k_apply
 constexpr int kTab = 512;                // per-wavefront LDS table slots
 constexpr int kTabMaxDeg = kTab * 3 / 4; // larger nodes use a global-memory table
-constexpr int kMaxMergeWaves = 16;
+constexpr int kMergeWaves = 4;           // wavefronts per workgroup in k_merge
+constexpr int kNBK = 512;                // bucket sums per row of M and M^T
+constexpr int kScr = 3 * kTab;           // per-wavefront LDS scratch (ints) for one bucket
+constexpr int kDegLds = 1024;            // k_degrees privatises its counters for B <= this
+constexpr int kSmallB = 128;             // k_insert_small: B*B counts in LDS
 constexpr int kEntropyBlocks = 1024;
 constexpr int kNumSnap = 4;
 
@@ -74,6 +80,24 @@ struct Status {
   int32_t pad;
 };
 
+// Device state of one nodal sweep (the reference's `update` task,
+// graph_parallel.hpp:433-458, run by the last block of each k_apply).
+struct Sweep {
+  unsigned long long cum_lo, cum_hi;  // exact cumulative dS of the sweep (int128)
+  double S;                           // running S: S += dS_sum after every batch
+  double mean;                        // moving average of earlier sweeps' dS
+  double thr;                         // delta_entropy_threshold1 or 2
+  double dS_last;                     // dS_sum after the last batch (itr_delta_entropy[itr])
+  unsigned long long moves;
+  unsigned long long nodal_bytes, apply_bytes;
+  int32_t cum_inf;  // an accepted dS was -inf
+  int32_t check;    // itr >= window - 1: the convergence test is active
+  int32_t stop;     // 0 run, 1 converged, 2 M needs a rebuild, 3 inconsistent M
+  int32_t done;     // batches executed
+  int32_t flags;    // OR of the batches' flags
+  uint32_t arrive;  // blocks of the current k_apply that finished
+};
+
 struct GraphView {
   int32_t N;
   const int64_t* out_ptr;
@@ -91,7 +115,9 @@ struct MView {
   int32_t* nkeys;
   int32_t* key;
   int32_t* val;
+  int32_t* bs;     // [B][kNBK]: sum of the row's counts per key bucket
   int32_t B;
+  int32_t bshift;  // bucket of key k = k >> bshift
 };
 
 __device__ __forceinline__ uint32_t hash32(int32_t b) {
@@ -119,9 +145,12 @@ __device__ __forceinline__ int32_t m_get(const MView& m, int32_t a, int32_t b) {
   return 0;
 }
 
-// M[a][b] += w (w < 0 only for a key that exists)
+// M[a][b] += w (w < 0 only for a key that exists); kBuckets also updates the
+// row's bucket sum
+template <bool kBuckets>
 __device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int32_t w,
                                       int32_t* flags) {
+  if (kBuckets) atomicAdd(&m.bs[(int64_t)a * kNBK + (b >> m.bshift)], w);
   const int64_t o = m.off[a];
   const int32_t c = m.cap[a];
   if (c == m.B) {
@@ -217,61 +246,132 @@ __device__ __forceinline__ i128 Tterm(const int64_t* lam, int64_t m, int64_t x, 
   return (i128)m * (i128)(lam[m] - lam[x] - lam[y]);
 }
 
-// per-lane part of sum_{key <= x, key != excl} M[a][key]
-__device__ __forceinline__ long count_le_lane(const MView& m, int64_t o, int32_t c, int32_t x,
-                                              int32_t excl) {
+// per-lane part of sum_{lo <= key <= x, key != excl} M[a][key]
+__device__ __forceinline__ long count_range_lane(const MView& m, int64_t o, int32_t c, int32_t lo,
+                                                 int32_t x, int32_t excl) {
   const int lane = lane_id();
   long acc = 0;
   if (c == m.B) {
-    for (int32_t i = lane; i <= x; i += kWave)
+    for (int32_t i = lo + lane; i <= x; i += kWave)
       if (i != excl) acc += m.val[o + i];
   } else {
     for (int32_t j = lane; j < c; j += kWave) {
       const int32_t k = m.key[o + j];
-      if (k >= 0 && k <= x && k != excl) acc += m.val[o + j];
+      if (k >= lo && k <= x && k != excl) acc += m.val[o + j];
     }
   }
   return acc;
 }
 
+// scr[k - lo] += M[a][k] for lo <= k < hi (keys of one row are distinct, so
+// each lane owns the scratch entries it touches)
+__device__ __forceinline__ void add_range(const MView& m, int64_t o, int32_t c, int32_t lo,
+                                          int32_t hi, int32_t* scr) {
+  const int lane = lane_id();
+  if (c == m.B) {
+    for (int32_t i = lo + lane; i < hi; i += kWave) scr[i - lo] += m.val[o + i];
+  } else {
+    for (int32_t j = lane; j < c; j += kWave) {
+      const int32_t k = m.key[o + j];
+      if (k >= lo && k < hi) scr[k - lo] += m.val[o + j];
+    }
+  }
+}
+
 // Inverse-CDF pick over the combined weights M[u][i] + M[i][u] (row u of M
 // and row u of M^T), ascending i, skipping i == excl: the first i whose
 // running sum exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact
-// integer weights).  The caller guarantees t < total weight.
+// integer weights).  The caller guarantees t < total weight.  Two levels: the
+// rows' bucket sums locate the key bucket, then the bucket's keys (at most
+// 2^bshift of them) are summed in ascending order, in the wavefront's LDS
+// scratch `scr` when a row is hashed.
 __device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
-                                   int32_t excl, long& bytes) {
+                                   int32_t excl, int32_t* scr, long& bytes) {
   const int lane = lane_id();
   const int32_t B = M.B;
+  const int sh = M.bshift;
   const int64_t oM = M.off[u], oT = Mt.off[u];
   const int32_t cM = M.cap[u], cT = Mt.cap[u];
-  bytes += 24;  // off and cap of both rows
+  const int32_t nbk = ((B - 1) >> sh) + 1;
+  int32_t xb = -1;
+  long wx = 0;
+  if (excl >= 0) {
+    xb = excl >> sh;
+    wx = (long)m_get(M, u, excl) + (long)m_get(Mt, u, excl);
+    bytes += 16;
+  }
+  // level 1: bucket of the t-th unit
+  const int32_t* bM = M.bs + (int64_t)u * kNBK;
+  const int32_t* bT = Mt.bs + (int64_t)u * kNBK;
+  int32_t jb = nbk - 1;
+  for (int32_t base = 0; base < nbk; base += kWave) {
+    const int32_t j = base + lane;
+    long w = 0;
+    if (j < nbk) {
+      w = (long)bM[j] + (long)bT[j];
+      if (j == xb) w -= wx;
+    }
+    const long inc = wave_incl_scan(w);
+    const long tot = __shfl(inc, kWave - 1);
+    bytes += 8 * (long)min(kWave, nbk - base);
+    if (t < tot) {
+      const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
+      jb = base + f;
+      t -= __shfl(inc, f) - __shfl(w, f);
+      break;
+    }
+    t -= tot;
+  }
+  // level 2: keys [lo, hi) of that bucket in ascending order
+  const int32_t lo = jb << sh;
+  const int32_t hi = min(B, lo + (1 << sh));
+  const int32_t width = hi - lo;
+  bytes += 24;
   if (cM == B && cT == B) {
-    for (int32_t base = 0; base < B; base += kWave) {
+    for (int32_t base = lo; base < hi; base += kWave) {
       const int32_t i = base + lane;
       long w = 0;
-      if (i < B && i != excl) w = (long)M.val[oM + i] + (long)Mt.val[oT + i];
+      if (i < hi && i != excl) w = (long)M.val[oM + i] + (long)Mt.val[oT + i];
       const long inc = wave_incl_scan(w);
       const long tot = __shfl(inc, kWave - 1);
-      bytes += 8 * (long)min(kWave, B - base);
-      if (t < tot) {
-        const unsigned long long hit = __ballot(inc > t);
-        return base + (__ffsll((unsigned long long)hit) - 1);
-      }
+      bytes += 8 * (long)min(kWave, hi - base);
+      if (t < tot) return base + (__ffsll((unsigned long long)__ballot(inc > t)) - 1);
       t -= tot;
     }
-    return B - 1;
+    return hi - 1;
   }
-  bytes += 8 * ((long)(cM == B ? B : cM) + (long)(cT == B ? B : cT));
-  int32_t lo = 0, hi = B - 1;
-  while (lo < hi) {
-    const int32_t mid = lo + (hi - lo) / 2;
-    const long c = wave_sum(count_le_lane(M, oM, cM, mid, excl) + count_le_lane(Mt, oT, cT, mid, excl));
+  bytes += 4 * ((long)(cM == B ? width : cM) + (long)(cT == B ? width : cT));
+  if (width <= kScr) {
+    for (int32_t i = lane; i < width; i += kWave) scr[i] = 0;
+    wave_fence();
+    add_range(M, oM, cM, lo, hi, scr);
+    wave_fence();
+    add_range(Mt, oT, cT, lo, hi, scr);
+    wave_fence();
+    if (lane == 0 && excl >= lo && excl < hi) scr[excl - lo] = 0;
+    wave_fence();
+    for (int32_t base = 0; base < width; base += kWave) {
+      const int32_t i = base + lane;
+      const long w = i < width ? (long)scr[i] : 0;
+      const long inc = wave_incl_scan(w);
+      const long tot = __shfl(inc, kWave - 1);
+      if (t < tot) return lo + base + (__ffsll((unsigned long long)__ballot(inc > t)) - 1);
+      t -= tot;
+    }
+    return hi - 1;
+  }
+  // buckets wider than the scratch (B > 786432): bisection inside the bucket
+  int32_t a = lo, b = hi - 1;
+  while (a < b) {
+    const int32_t mid = a + (b - a) / 2;
+    const long c = wave_sum(count_range_lane(M, oM, cM, lo, mid, excl) +
+                            count_range_lane(Mt, oT, cT, lo, mid, excl));
     if (c > t)
-      hi = mid;
+      b = mid;
     else
-      lo = mid + 1;
+      a = mid + 1;
   }
-  return lo;
+  return a;
 }
 
 // t-th unit of the node's neighbour list (out-neighbours then in-neighbours,
@@ -305,38 +405,60 @@ __device__ int32_t pick_neighbor(const GraphView& g, int64_t o0, int64_t kco, in
 // M build
 // ---------------------------------------------------------------------------
 
-__global__ void k_degrees(GraphView g, const int32_t* __restrict__ labels, int32_t* d_out,
-                          int32_t* d_in, int32_t* ecnt_out, int32_t* ecnt_in, int32_t* bsize) {
-  const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= g.N) return;
-  const int32_t a = labels[x];
-  const int64_t o0 = g.out_ptr[x], o1 = g.out_ptr[x + 1];
-  const int64_t i0 = g.in_ptr[x], i1 = g.in_ptr[x + 1];
-  int64_t wo = o1 - o0, wi = i1 - i0;
-  if (g.out_w) {
-    wo = 0;
-    wi = 0;
-    for (int64_t e = o0; e < o1; ++e) wo += g.out_w[e];
-    for (int64_t e = i0; e < i1; ++e) wi += g.in_w[e];
+// Block degrees, edge counts and node counts from the labels; counters are
+// privatised in LDS when B is small (many nodes share a block).
+__global__ __launch_bounds__(256) void k_degrees(GraphView g, const int32_t* __restrict__ labels,
+                                                 int32_t B, int32_t* d_out, int32_t* d_in,
+                                                 int32_t* ecnt_out, int32_t* ecnt_in,
+                                                 int32_t* bsize) {
+  __shared__ int32_t s_c[5][kDegLds];
+  const bool lds = B <= kDegLds;
+  if (lds) {
+    for (int i = threadIdx.x; i < 5 * kDegLds; i += blockDim.x) (&s_c[0][0])[i] = 0;
+    __syncthreads();
   }
-  if (wo) atomicAdd(&d_out[a], (int32_t)wo);
-  if (wi) atomicAdd(&d_in[a], (int32_t)wi);
-  if (o1 > o0) atomicAdd(&ecnt_out[a], (int32_t)(o1 - o0));
-  if (i1 > i0) atomicAdd(&ecnt_in[a], (int32_t)(i1 - i0));
-  atomicAdd(&bsize[a], 1);
+  int32_t* dst[5] = {d_out, d_in, ecnt_out, ecnt_in, bsize};
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < g.N;
+       x += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t a = labels[x];
+    const int64_t o0 = g.out_ptr[x], o1 = g.out_ptr[x + 1];
+    const int64_t i0 = g.in_ptr[x], i1 = g.in_ptr[x + 1];
+    int64_t wo = o1 - o0, wi = i1 - i0;
+    if (g.out_w) {
+      wo = 0;
+      wi = 0;
+      for (int64_t e = o0; e < o1; ++e) wo += g.out_w[e];
+      for (int64_t e = i0; e < i1; ++e) wi += g.in_w[e];
+    }
+    const int32_t v[5] = {(int32_t)wo, (int32_t)wi, (int32_t)(o1 - o0), (int32_t)(i1 - i0), 1};
+    for (int k = 0; k < 5; ++k) {
+      if (!v[k]) continue;
+      if (lds)
+        atomicAdd(&s_c[k][a], v[k]);
+      else
+        atomicAdd(&dst[k][a], v[k]);
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int32_t b = threadIdx.x; b < B; b += blockDim.x)
+      for (int k = 0; k < 5; ++k)
+        if (s_c[k][b]) atomicAdd(&dst[k][b], s_c[k][b]);
+  }
 }
 
-// Slab capacity of a row with ecnt edges: a power of two >= 4*min(B, ecnt)+16
+// Slab capacity of a row with ecnt edges: a power of two >= slack*min(B, ecnt)+16
 // (so keys inserted by later moves fit without a rebuild), or a direct row of
-// B slots when that is no larger.
-__device__ __forceinline__ int32_t cap_for(int32_t B, int32_t ecnt) {
+// B slots once that would reach B/8 (bounds a hashed row's scan; direct rows
+// total at most 8*slack*E slots).
+__device__ __forceinline__ int32_t cap_for(int32_t B, int32_t ecnt, int32_t slack) {
   const int64_t bound = ecnt < B ? ecnt : B;
   int64_t c = 32;
-  while (c < 4 * bound + 16) c <<= 1;
-  return c >= B ? B : (int32_t)c;
+  while (c < slack * bound + 16) c <<= 1;
+  return 8 * c >= B ? B : (int32_t)c;
 }
 
-__global__ void k_caps(int32_t B, const int32_t* ecnt_out, const int32_t* ecnt_in,
+__global__ void k_caps(int32_t B, int32_t slack, const int32_t* ecnt_out, const int32_t* ecnt_in,
                        int32_t* capM, int32_t* capT, int64_t* tmpM, int64_t* tmpT) {
   const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a > B) return;
@@ -345,8 +467,8 @@ __global__ void k_caps(int32_t B, const int32_t* ecnt_out, const int32_t* ecnt_i
     tmpT[B] = 0;
     return;
   }
-  const int32_t cm = cap_for(B, ecnt_out[a]);
-  const int32_t ct = cap_for(B, ecnt_in[a]);
+  const int32_t cm = cap_for(B, ecnt_out[a], slack);
+  const int32_t ct = cap_for(B, ecnt_in[a], slack);
   capM[a] = cm;
   capT[a] = ct;
   tmpM[a] = cm;
@@ -370,9 +492,54 @@ __global__ void k_insert_edges(int64_t E, const int32_t* __restrict__ src,
     const int32_t a = labels[src[e]];
     const int32_t b = labels[dst[e]];
     const int32_t ww = w ? w[e] : 1;
-    m_add(M, a, b, ww, flags);
-    m_add(Mt, b, a, ww, flags);
+    m_add<false>(M, a, b, ww, flags);
+    m_add<false>(Mt, b, a, ww, flags);
   }
+}
+
+// M build for B <= kSmallB (every row direct): per-workgroup B*B counts in
+// LDS, flushed once, so the few hot cells are not hammered by atomics.
+__global__ __launch_bounds__(256) void k_insert_small(int64_t E, const int32_t* __restrict__ src,
+                                                      const int32_t* __restrict__ dst,
+                                                      const int32_t* __restrict__ w,
+                                                      const int32_t* __restrict__ labels,
+                                                      MView M, MView Mt) {
+  __shared__ int32_t s_h[kSmallB * kSmallB];
+  const int32_t B = M.B;
+  for (int i = threadIdx.x; i < B * B; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&s_h[labels[src[e]] * B + labels[dst[e]]], w ? w[e] : 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < B * B; i += blockDim.x) {
+    const int32_t v = s_h[i];
+    if (!v) continue;
+    const int32_t a = i / B, b = i - a * B;
+    atomicAdd(&M.val[M.off[a] + b], v);
+    atomicAdd(&Mt.val[Mt.off[b] + a], v);
+  }
+}
+
+// Bucket sums of every row (one wavefront per row).
+__global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
+  __shared__ int32_t s_b[4][kNBK];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const int64_t a = (int64_t)blockIdx.x * 4 + w;
+  if (a >= m.B) return;
+  int32_t* bk = s_b[w];
+  for (int i = lane; i < kNBK; i += kWave) bk[i] = 0;
+  wave_fence();
+  const int64_t o = m.off[a];
+  const int32_t c = m.cap[a];
+  const bool direct = c == m.B;
+  for (int32_t j = lane; j < c; j += kWave) {
+    const int32_t k = direct ? j : m.key[o + j];
+    const int32_t v = m.val[o + j];
+    if (k >= 0 && v != 0) atomicAdd(&bk[k >> m.bshift], v);
+  }
+  wave_fence();
+  for (int i = lane; i < kNBK; i += kWave) m.bs[a * kNBK + i] = bk[i];
 }
 
 __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
@@ -457,8 +624,8 @@ __device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r
   return acc;
 }
 
-__device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t& s_out,
-                          i128& dS_out) {
+__device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* scr,
+                          int32_t& s_out, i128& dS_out) {
   const int32_t B = a.B;
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   const int64_t k = dor + dir;
@@ -469,14 +636,14 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t& s
     s = (int32_t)usap::idx_of(x[2], B);
   } else {
     long bytes = 0;
-    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1, bytes);
+    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1, scr, bytes);
     const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
     if (usap::uniform_branch(x[1], da, B)) {
       s = usap::uniform_not_r(x[2], r, B);
     } else {
       const int64_t W2 = da - m_get(a.M, u, r) - m_get(a.Mt, u, r);
       s = W2 == 0 ? usap::uniform_not_r(x[2], r, B)
-                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r, bytes);
+                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r, scr, bytes);
     }
   }
   s_out = s;
@@ -503,11 +670,12 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t& s
   dS_out = old_c - new_c + acc + deg;
 }
 
-__global__ void k_merge(MergeArgs a) {
-  __shared__ long s_hi[kMaxMergeWaves];
-  __shared__ unsigned long s_lo[kMaxMergeWaves];
-  __shared__ int32_t s_idx[kMaxMergeWaves];
-  __shared__ int32_t s_s[kMaxMergeWaves];
+__global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
+  __shared__ long s_hi[kMergeWaves];
+  __shared__ unsigned long s_lo[kMergeWaves];
+  __shared__ int32_t s_idx[kMergeWaves];
+  __shared__ int32_t s_s[kMergeWaves];
+  __shared__ int32_t s_scr[kMergeWaves][kScr];
   const int32_t r = blockIdx.x;
   const int w = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
@@ -516,7 +684,7 @@ __global__ void k_merge(MergeArgs a) {
   for (int32_t idx = w; idx < a.K; idx += nw) {
     int32_t s;
     i128 d;
-    merge_one(a, r, idx, s, d);
+    merge_one(a, r, idx, s_scr[w], s, d);
     if (bidx < 0 || d < bd) {  // strict less: the earliest proposal wins ties
       bidx = idx;
       bs = s;
@@ -569,7 +737,9 @@ struct NodalArgs {
   unsigned long* prop_lo;
   int32_t* prop_inf;
   int32_t* prop_acc;
+  int32_t* prop_r;
   Status* status;
+  const Sweep* sweep;
 };
 
 __device__ __forceinline__ void tab_add(int32_t* tk, int32_t* tv, uint32_t mask, int32_t b,
@@ -703,7 +873,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
-  if (n64 >= a.end) return;
+  if (n64 >= a.end || a.sweep->stop) return;
   const int32_t n = (int32_t)n64;
   const int32_t B = a.B;
   const int32_t r = a.labels[n];
@@ -734,7 +904,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     if (usap::uniform_branch(x[1], da, B))
       s = (int32_t)usap::idx_of(x[2], B);
     else
-      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1, bytes);
+      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1, &s_tab[w][0][0], bytes);
   }
   i128 dS = 0;
   bool inf = false;
@@ -761,7 +931,8 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     a.prop_lo[item] = (unsigned long)(u128)dS;
     a.prop_inf[item] = inf ? -1 : 0;
     a.prop_acc[item] = acc;
-    atomicAdd(&a.status->nodal_bytes, (unsigned long long)bytes);
+    a.prop_r[item] = r;
+    atomicAdd(&a.status->nodal_bytes, (unsigned long long)bytes + 4);
   }
 }
 
@@ -773,7 +944,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
 struct ApplyArgs {
   GraphView g;
   MView M, Mt;
-  const int32_t* labels;
+  int32_t* labels;
   int32_t* d_out;
   int32_t* d_in;
   int32_t* bsize;
@@ -783,90 +954,146 @@ struct ApplyArgs {
   const unsigned long* prop_lo;
   const int32_t* prop_inf;
   const int32_t* prop_acc;
+  const int32_t* prop_r;
   Status* status;
+  Sweep* sweep;
 };
 
+__device__ double fx_to_double_dev(i128 v) {
+  const bool neg = v < 0;
+  const u128 m = neg ? (u128)0 - (u128)v : (u128)v;
+  const unsigned long long hi = (unsigned long long)(m >> 64), lo = (unsigned long long)m;
+  const double d = ldexp((double)hi, 64 - usap::kLambdaFrac) + ldexp((double)lo, -usap::kLambdaFrac);
+  return neg ? -d : d;
+}
+
+// graph_parallel.hpp:433-458 after the moves of one batch: cumulative dS of
+// the sweep, S += dS_sum, the moving-average convergence test; clears the
+// batch status.  Same IEEE double operations as the host driver.
+__device__ void sweep_tail(Sweep* sw, Status* st) {
+  const unsigned long long lo = __hip_atomic_load(&st->sum_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long hi = __hip_atomic_load(&st->sum_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t ninf = __hip_atomic_load(&st->ninf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t moves = __hip_atomic_load(&st->moves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t flags = __hip_atomic_load(&st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  sw->nodal_bytes += __hip_atomic_load(&st->nodal_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  sw->apply_bytes += __hip_atomic_load(&st->apply_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const u128 cum = (((u128)sw->cum_hi << 64) | sw->cum_lo) + (((u128)hi << 64) | lo);
+  sw->cum_lo = (unsigned long long)cum;
+  sw->cum_hi = (unsigned long long)(cum >> 64);
+  if (ninf > 0) sw->cum_inf = 1;
+  const double dS_sum = sw->cum_inf ? -INFINITY : fx_to_double_dev((i128)cum);
+  sw->dS_last = dS_sum;
+  sw->S += dS_sum;
+  sw->done += 1;
+  sw->moves += (unsigned long long)moves;
+  sw->flags |= flags;
+  if (flags & (kFlagMissing | kFlagNegative))
+    sw->stop = 3;
+  else if (sw->check && -sw->mean < sw->thr * sw->S)
+    sw->stop = 1;
+  else if (flags & (kFlagLoad | kFlagFull))
+    sw->stop = 2;
+  st->sum_lo = 0;
+  st->sum_hi = 0;
+  st->nodal_bytes = 0;
+  st->apply_bytes = 0;
+  st->ninf = 0;
+  st->moves = 0;
+  st->flags = 0;
+}
+
+// Moves of the accepted nodes of one batch, then their labels.  Old labels of
+// nodes inside the batch come from prop_r (k_nodal stored them), so writing a
+// moved node's label cannot race with a neighbour's read.
 __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
+  __shared__ int s_last;
+  if (a.sweep->stop) return;  // the same for every block of this launch
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
-  if (n64 >= a.end) return;
-  if (!a.prop_acc[item]) return;
-  const int32_t n = (int32_t)n64;
-  const int32_t r = a.labels[n];
-  const int32_t s = a.prop_s[item];
-  int32_t* flags = &a.status->flags;
-  const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
-  const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
-  long wo_sum = 0, wi_sum = 0;
-  for (int64_t e = o0 + lane; e < o1; e += kWave) {
-    const int32_t v = a.g.out_idx[e];
-    const int32_t ww = a.g.out_w ? a.g.out_w[e] : 1;
-    const int32_t bo = a.labels[v];
-    int32_t bn = bo;
-    if (v >= a.beg && v < a.end && a.prop_acc[v - a.beg]) bn = a.prop_s[v - a.beg];
-    m_add(a.M, r, bo, -ww, flags);
-    m_add(a.Mt, bo, r, -ww, flags);
-    m_add(a.M, s, bn, ww, flags);
-    m_add(a.Mt, bn, s, ww, flags);
-    wo_sum += ww;
-  }
-  for (int64_t e = i0 + lane; e < i1; e += kWave) {
-    const int32_t v = a.g.in_idx[e];
-    const int32_t ww = a.g.in_w ? a.g.in_w[e] : 1;
-    wi_sum += ww;
-    if (v >= a.beg && v < a.end && a.prop_acc[v - a.beg]) continue;  // done as v's out-edge
-    const int32_t b = a.labels[v];
-    m_add(a.M, b, r, -ww, flags);
-    m_add(a.Mt, r, b, -ww, flags);
-    m_add(a.M, b, s, ww, flags);
-    m_add(a.Mt, s, b, ww, flags);
-  }
-  wo_sum = wave_sum(wo_sum);
-  wi_sum = wave_sum(wi_sum);
-  if (lane == 0) {
-    atomicAdd(&a.status->apply_bytes,
-              (unsigned long long)(44 + 60 * ((o1 - o0) + (i1 - i0)) + 32 + 32));
-    if (wo_sum) {
-      atomicSub(&a.d_out[r], (int32_t)wo_sum);
-      atomicAdd(&a.d_out[s], (int32_t)wo_sum);
+  if (n64 < a.end && a.prop_acc[item]) {
+    const int32_t n = (int32_t)n64;
+    const int32_t r = a.prop_r[item];
+    const int32_t s = a.prop_s[item];
+    int32_t* flags = &a.status->flags;
+    const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
+    const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
+    long wo_sum = 0, wi_sum = 0;
+    for (int64_t e = o0 + lane; e < o1; e += kWave) {
+      const int32_t v = a.g.out_idx[e];
+      const int32_t ww = a.g.out_w ? a.g.out_w[e] : 1;
+      int32_t bo, bn;
+      if (v >= a.beg && v < a.end) {
+        bo = a.prop_r[v - a.beg];
+        bn = a.prop_acc[v - a.beg] ? a.prop_s[v - a.beg] : bo;
+      } else {
+        bo = bn = a.labels[v];
+      }
+      m_add<true>(a.M, r, bo, -ww, flags);
+      m_add<true>(a.Mt, bo, r, -ww, flags);
+      m_add<true>(a.M, s, bn, ww, flags);
+      m_add<true>(a.Mt, bn, s, ww, flags);
+      wo_sum += ww;
     }
-    if (wi_sum) {
-      atomicSub(&a.d_in[r], (int32_t)wi_sum);
-      atomicAdd(&a.d_in[s], (int32_t)wi_sum);
+    for (int64_t e = i0 + lane; e < i1; e += kWave) {
+      const int32_t v = a.g.in_idx[e];
+      const int32_t ww = a.g.in_w ? a.g.in_w[e] : 1;
+      wi_sum += ww;
+      int32_t b;
+      if (v >= a.beg && v < a.end) {
+        if (a.prop_acc[v - a.beg]) continue;  // moved as v's out-edge
+        b = a.prop_r[v - a.beg];
+      } else {
+        b = a.labels[v];
+      }
+      m_add<true>(a.M, b, r, -ww, flags);
+      m_add<true>(a.Mt, r, b, -ww, flags);
+      m_add<true>(a.M, b, s, ww, flags);
+      m_add<true>(a.Mt, s, b, ww, flags);
     }
-    atomicSub(&a.bsize[r], 1);
-    atomicAdd(&a.bsize[s], 1);
-    atomicAdd(&a.status->moves, 1);
-    if (a.prop_inf[item]) {
-      atomicAdd(&a.status->ninf, 1);
-    } else {
-      // int128 add as two 64-bit atomics with an explicit carry: exact once
-      // every wave has added, in any order
-      const unsigned long lo = a.prop_lo[item];
-      const unsigned long long old = atomicAdd(&a.status->sum_lo, (unsigned long long)lo);
-      const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-      atomicAdd(&a.status->sum_hi, (unsigned long long)a.prop_hi[item] + carry);
+    wo_sum = wave_sum(wo_sum);
+    wi_sum = wave_sum(wi_sum);
+    if (lane == 0) {
+      atomicAdd(&a.status->apply_bytes,
+                (unsigned long long)(52 + 60 * ((o1 - o0) + (i1 - i0)) + 32 + 32 + 4));
+      if (wo_sum) {
+        atomicSub(&a.d_out[r], (int32_t)wo_sum);
+        atomicAdd(&a.d_out[s], (int32_t)wo_sum);
+      }
+      if (wi_sum) {
+        atomicSub(&a.d_in[r], (int32_t)wi_sum);
+        atomicAdd(&a.d_in[s], (int32_t)wi_sum);
+      }
+      atomicSub(&a.bsize[r], 1);
+      atomicAdd(&a.bsize[s], 1);
+      atomicAdd(&a.status->moves, 1);
+      if (a.prop_inf[item]) {
+        atomicAdd(&a.status->ninf, 1);
+      } else {
+        // int128 add as two 64-bit atomics with an explicit carry: exact once
+        // every wave has added, in any order
+        const unsigned long lo = a.prop_lo[item];
+        const unsigned long long old = atomicAdd(&a.status->sum_lo, (unsigned long long)lo);
+        const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+        atomicAdd(&a.status->sum_hi, (unsigned long long)a.prop_hi[item] + carry);
+      }
+      a.labels[n] = s;
     }
   }
-}
-
-// Also clears the status buffer of the next batch (statuses alternate, so no
-// kernel of this batch touches it).
-__global__ void k_labels(int32_t beg, int32_t end, int32_t* labels, const int32_t* prop_s,
-                         const int32_t* prop_acc, Status* next_status) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    next_status->sum_lo = 0;
-    next_status->sum_hi = 0;
-    next_status->nodal_bytes = 0;
-    next_status->apply_bytes = 0;
-    next_status->ninf = 0;
-    next_status->moves = 0;
-    next_status->flags = 0;
+  // the last block to finish runs the sweep step
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(&a.sweep->arrive, 1u) == gridDim.x - 1;
   }
-  if ((int64_t)beg + i >= end) return;
-  if (prop_acc[i]) labels[beg + i] = prop_s[i];
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    sweep_tail(a.sweep, a.status);
+    a.sweep->arrive = 0;
+    __threadfence();
+  }
 }
 
 }  // namespace
@@ -881,6 +1108,8 @@ struct DevSpM {
   int32_t* nkeys = nullptr;
   int32_t* key = nullptr;
   int32_t* val = nullptr;
+  int32_t* bs = nullptr;  // [bs_rows][kNBK]
+  int32_t bs_rows = 0;
   int64_t P = 0;      // allocated slots
   int64_t total = 0;  // slots in use
 };
@@ -925,9 +1154,11 @@ struct usap_engine {
   int32_t* prop_inf = nullptr;
   int32_t* prop_acc = nullptr;
   int32_t last_beg = 0, last_end = 0;
-  Status* status = nullptr;  // [2], alternating per batch
-  int32_t status_cur = 0;
-  Status* h_status = nullptr;
+  Status* status = nullptr;   // the running batch
+  Sweep* sweep = nullptr;     // device sweep state
+  Sweep* h_sweep = nullptr;   // pinned host copy
+  int32_t* prop_r = nullptr;  // old label of each batch node
+  std::vector<hipEvent_t> bev;  // per-batch kernel events (profiling)
   int32_t* build_flags = nullptr;
   int64_t* big_off = nullptr;
   int32_t* big_tab = nullptr;
@@ -980,6 +1211,12 @@ void dfree(T*& p) {
 
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+int32_t bucket_shift(int32_t B) {
+  int32_t sh = 0;
+  while (((B - 1) >> sh) >= kNBK) ++sh;
+  return sh;
+}
+
 MView view(const DevSpM& m, int32_t B) {
   MView v;
   v.off = m.off;
@@ -987,7 +1224,9 @@ MView view(const DevSpM& m, int32_t B) {
   v.nkeys = m.nkeys;
   v.key = m.key;
   v.val = m.val;
+  v.bs = m.bs;
   v.B = B;
+  v.bshift = bucket_shift(B);
   return v;
 }
 
@@ -1042,10 +1281,13 @@ int rebuild_M(usap_engine* e) {
   HIPCHK(hipMemsetAsync(e->ecnt_out, 0, sizeof(int32_t) * B, st));
   HIPCHK(hipMemsetAsync(e->ecnt_in, 0, sizeof(int32_t) * B, st));
   HIPCHK(hipMemsetAsync(e->bsize, 0, sizeof(int32_t) * B, st));
-  if (e->N > 0)
-    k_degrees<<<blocks_for(e->N, 256), 256, 0, st>>>(gview(e), e->labels, e->d_out, e->d_in,
-                                                      e->ecnt_out, e->ecnt_in, e->bsize);
-  k_caps<<<blocks_for((int64_t)B + 1, 256), 256, 0, st>>>(B, e->ecnt_out, e->ecnt_in, e->M.cap,
+  if (e->N > 0) {
+    const unsigned g = std::min<unsigned>(blocks_for(e->N, 256), B <= kDegLds ? 256u : 4096u);
+    k_degrees<<<g, 256, 0, st>>>(gview(e), e->labels, B, e->d_out, e->d_in, e->ecnt_out,
+                                 e->ecnt_in, e->bsize);
+  }
+  k_caps<<<blocks_for((int64_t)B + 1, 256), 256, 0, st>>>(B, e->p.slab_slack, e->ecnt_out,
+                                                          e->ecnt_in, e->M.cap,
                                                           e->Mt.cap, e->tmpM, e->tmpT);
   HIPCHK(hipGetLastError());
   size_t bytes = 0;
@@ -1081,12 +1323,24 @@ int rebuild_M(usap_engine* e) {
     if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
         m.key, m.val, m.total);
     HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
+    if (m.bs_rows < B) {
+      dfree(m.bs);
+      if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
+      m.bs_rows = B;
+    }
   }
   HIPCHK(hipMemsetAsync(e->build_flags, 0, sizeof(int32_t), st));
-  if (e->E > 0)
-    k_insert_edges<<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
-        e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), view(e->Mt, B),
-        e->build_flags);
+  if (e->E > 0) {
+    if (B <= kSmallB)
+      k_insert_small<<<256, 256, 0, st>>>(e->E, e->out_src, e->out_idx, e->out_w, e->labels,
+                                          view(e->M, B), view(e->Mt, B));
+    else
+      k_insert_edges<<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
+          e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), view(e->Mt, B),
+          e->build_flags);
+  }
+  k_bucket_sums<<<blocks_for(B, 4), 256, 0, st>>>(view(e->M, B));
+  k_bucket_sums<<<blocks_for(B, 4), 256, 0, st>>>(view(e->Mt, B));
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
   int32_t flags = 0;
@@ -1132,6 +1386,7 @@ void usap_default_params(usap_params* p) {
   p->seed = 0;
   p->device = 0;
   p->verbose = 0;
+  p->slab_slack = 4;
 }
 
 const char* usap_last_error(const usap_engine* e) { return e ? e->err.c_str() : "null engine"; }
@@ -1159,6 +1414,7 @@ int usap_destroy(usap_engine* e) {
     dfree(m->nkeys);
     dfree(m->key);
     dfree(m->val);
+    dfree(m->bs);
   }
   dfree(e->tmpM);
   dfree(e->tmpT);
@@ -1173,8 +1429,11 @@ int usap_destroy(usap_engine* e) {
   dfree(e->prop_inf);
   dfree(e->prop_acc);
   dfree(e->status);
+  dfree(e->sweep);
+  dfree(e->prop_r);
   dfree(e->build_flags);
-  if (e->h_status) (void)hipHostFree(e->h_status);
+  for (hipEvent_t ev : e->bev) (void)hipEventDestroy(ev);
+  if (e->h_sweep) (void)hipHostFree(e->h_sweep);
   dfree(e->big_off);
   dfree(e->big_tab);
   dfree(e->ent_part);
@@ -1204,7 +1463,7 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
   else
     usap_default_params(&e->p);
   if (e->p.K < 1 || e->p.reduction_rate <= 0 || e->p.reduction_rate >= 1 || e->p.avg_window < 2 ||
-      e->p.max_nodal_itr < 1) {
+      e->p.max_nodal_itr < 1 || e->p.slab_slack < 1 || e->p.slab_slack > 16) {
     delete e;
     return USAP_EINVAL;
   }
@@ -1306,10 +1565,12 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     TRY(dalloc(e, &e->prop_lo, N));
     TRY(dalloc(e, &e->prop_inf, N));
     TRY(dalloc(e, &e->prop_acc, N));
-    TRY(dalloc(e, &e->status, 2));
+    TRY(dalloc(e, &e->status, 1));
+    TRY(dalloc(e, &e->sweep, 1));
+    TRY(dalloc(e, &e->prop_r, N));
     TRY(dalloc(e, &e->build_flags, 1));
     TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
-    if (hipHostMalloc((void**)&e->h_status, sizeof(Status)) != hipSuccess ||
+    if (hipHostMalloc((void**)&e->h_sweep, sizeof(Sweep)) != hipSuccess ||
         hipHostMalloc((void**)&e->h_ent_part, sizeof(long) * 2 * kEntropyBlocks) != hipSuccess) {
       rc = set_err(e, USAP_ENOMEM, "hipHostMalloc");
       goto fail;
@@ -1348,7 +1609,9 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     }
     TRYH(hipMemcpyAsync(e->lam, lam.data(), sizeof(int64_t) * (wtot + 1), hipMemcpyHostToDevice, st));
     TRYH(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, st));
-    TRYH(hipMemsetAsync(e->status, 0, 2 * sizeof(Status), st));
+    TRYH(hipMemsetAsync(e->status, 0, sizeof(Status), st));
+    TRYH(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
+    std::memset(e->h_sweep, 0, sizeof(Sweep));
     for (hipEvent_t& ev : e->ev) TRYH(hipEventCreate(&ev));
     TRYH(hipStreamSynchronize(st));
   }
@@ -1477,7 +1740,7 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   a.best_s = e->best_s;
   a.best_hi = e->best_hi;
   a.best_lo = e->best_lo;
-  const int nw = std::min(K, kMaxMergeWaves);
+  const int nw = std::min(K, kMergeWaves);
   ev_rec(e, 0);
   k_merge<<<B, nw * kWave, 0, e->stream>>>(a);
   HIPCHK(hipGetLastError());
@@ -1577,21 +1840,14 @@ int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new) {
   return rebuild_M(e);
 }
 
-int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream, double* dS_sum,
-                     int32_t* n_moves) {
-  if (int rc = check_engine(e)) return rc;
-  if (begin < 0 || end > e->N || begin > end) return USAP_EINVAL;
-  hipStream_t st = e->stream;
-  e->last_beg = begin;
-  e->last_end = end;
-  if (begin == end) {
-    std::memset(e->h_status, 0, sizeof(Status));
-    if (dS_sum) *dS_sum = 0;
-    if (n_moves) *n_moves = 0;
-    return USAP_OK;
-  }
+}  // extern "C"
+
+namespace {
+
+// Launches k_nodal + k_apply for nodes [beg, end) with random stream `stream`.
+int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size_t ev_slot) {
   const int32_t B = e->B;
-  const int64_t n = end - begin;
+  hipStream_t st = e->stream;
   NodalArgs a;
   a.g = gview(e);
   a.M = view(e->M, B);
@@ -1601,7 +1857,7 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
   a.d_in = e->d_in;
   a.lam = e->lam;
   a.B = B;
-  a.beg = begin;
+  a.beg = beg;
   a.end = end;
   a.exact = e->p.ds_mode == USAP_DS_EXACT;
   a.seed = e->p.seed;
@@ -1613,13 +1869,9 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
   a.prop_lo = e->prop_lo;
   a.prop_inf = e->prop_inf;
   a.prop_acc = e->prop_acc;
-  Status* cur = e->status + e->status_cur;
-  Status* nxt = e->status + (e->status_cur ^ 1);
-  a.status = cur;
-  const unsigned grid = blocks_for(n, kNodalWaves);
-  ev_rec(e, 0);
-  k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
-  ev_rec(e, 1);
+  a.prop_r = e->prop_r;
+  a.status = e->status;
+  a.sweep = e->sweep;
   ApplyArgs b;
   b.g = a.g;
   b.M = a.M;
@@ -1628,45 +1880,159 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
   b.d_out = e->d_out;
   b.d_in = e->d_in;
   b.bsize = e->bsize;
-  b.beg = begin;
+  b.beg = beg;
   b.end = end;
   b.prop_s = e->prop_s;
   b.prop_hi = e->prop_hi;
   b.prop_lo = e->prop_lo;
   b.prop_inf = e->prop_inf;
   b.prop_acc = e->prop_acc;
-  b.status = cur;
+  b.prop_r = e->prop_r;
+  b.status = e->status;
+  b.sweep = e->sweep;
+  const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
+  if (e->prof) {
+    while (e->bev.size() < 3 * (ev_slot + 1)) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      e->bev.push_back(ev);
+    }
+    HIPCHK(hipEventRecord(e->bev[3 * ev_slot], st));
+  }
+  k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
+  if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 1], st));
   k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
-  k_labels<<<blocks_for(n, 256), 256, 0, st>>>(begin, end, e->labels, e->prop_s, e->prop_acc, nxt);
+  if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 2], st));
   HIPCHK(hipGetLastError());
-  ev_rec(e, 2);
-  HIPCHK(hipMemcpyAsync(e->h_status, cur, sizeof(Status), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  e->status_cur ^= 1;
-  const Status& s = *e->h_status;
-  prof_add(e, USAP_K_NODAL, ev_ms(e, 0, 1), 1, (double)s.nodal_bytes);
-  prof_add(e, USAP_K_APPLY, ev_ms(e, 1, 2), 2, (double)s.apply_bytes + 4.0 * n);
-  if (s.flags & (kFlagMissing | kFlagNegative))
-    return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
-  if (s.flags & (kFlagLoad | kFlagFull)) {
-    if (int rc = rebuild_M(e)) return rc;
+  return USAP_OK;
+}
+
+// Runs the batches [first*batch, N) of a sweep from the device state already
+// in e->sweep, one host synchronisation at the end; rebuilds M and resumes
+// when a batch overflowed a row slab.  Stops early when the sweep's
+// convergence test fires.
+int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, uint64_t stream0,
+              int32_t* rebuilds) {
+  hipStream_t st = e->stream;
+  Sweep& hs = *e->h_sweep;
+  int32_t next = first;
+  for (;;) {
+    size_t slot = 0;
+    for (int32_t i = next; i < last; ++i, ++slot) {
+      const int32_t beg = i * batch;
+      const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
+      if (int rc = launch_batch(e, beg, end, stream0 + (uint64_t)i, slot)) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const int32_t executed = first + hs.done - next;  // batches run in this launch round
+    if (e->prof) {
+      for (int32_t i = 0; i < executed; ++i) {
+        float a = 0.f, b = 0.f;
+        (void)hipEventElapsedTime(&a, e->bev[3 * i], e->bev[3 * i + 1]);
+        (void)hipEventElapsedTime(&b, e->bev[3 * i + 1], e->bev[3 * i + 2]);
+        e->kp[USAP_K_NODAL].ms += a;
+        e->kp[USAP_K_NODAL].launches += 1;
+        e->kp[USAP_K_APPLY].ms += b;
+        e->kp[USAP_K_APPLY].launches += 1;
+      }
+    }
+    if (hs.stop == 3) return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
+    if (hs.flags & (kFlagLoad | kFlagFull)) {
+      if (int rc = rebuild_M(e)) return rc;
+      if (rebuilds) ++*rebuilds;
+    }
+    if (hs.stop != 2 || first + hs.done >= last) break;
+    // resume after the rebuild
+    next = first + hs.done;
+    hs.stop = 0;
+    hs.flags = 0;
+    HIPCHK(hipMemcpyAsync(e->sweep, &hs, sizeof(Sweep), hipMemcpyHostToDevice, st));
   }
-  if (dS_sum) {
-    if (s.ninf > 0)
-      *dS_sum = -INFINITY;
-    else
-      *dS_sum = fx_to_double(mk128((long)s.sum_hi, (unsigned long)s.sum_lo));
+  if (e->prof) {
+    e->kp[USAP_K_NODAL].bytes += (double)hs.nodal_bytes;
+    e->kp[USAP_K_APPLY].bytes += (double)hs.apply_bytes;
   }
-  if (n_moves) *n_moves = s.moves;
+  return USAP_OK;
+}
+
+void init_sweep(usap_engine* e, double S, double mean, double thr, int32_t check) {
+  Sweep& hs = *e->h_sweep;
+  std::memset(&hs, 0, sizeof(Sweep));
+  hs.S = S;
+  hs.mean = mean;
+  hs.thr = thr;
+  hs.check = check;
+}
+
+}  // namespace
+
+extern "C" {
+
+int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream, double* dS_sum,
+                     int32_t* n_moves) {
+  if (int rc = check_engine(e)) return rc;
+  if (begin < 0 || end > e->N || begin > end) return USAP_EINVAL;
+  e->last_beg = begin;
+  e->last_end = end;
+  init_sweep(e, 0.0, 0.0, 0.0, 0);
+  if (begin < end) {
+    HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
+    // one batch of size end-begin starting at begin: index 0 of a sweep whose
+    // "batch" starts at begin
+    if (int rc = launch_batch(e, begin, end, stream, 0)) return rc;
+    Sweep& hs = *e->h_sweep;
+    HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->prof) {
+      float a = 0.f, b = 0.f;
+      (void)hipEventElapsedTime(&a, e->bev[0], e->bev[1]);
+      (void)hipEventElapsedTime(&b, e->bev[1], e->bev[2]);
+      prof_add(e, USAP_K_NODAL, a, 1, (double)hs.nodal_bytes);
+      prof_add(e, USAP_K_APPLY, b, 1, (double)hs.apply_bytes);
+    }
+    if (hs.stop == 3) return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
+    if (hs.flags & (kFlagLoad | kFlagFull))
+      if (int rc = rebuild_M(e)) return rc;
+  }
+  const Sweep& hs = *e->h_sweep;
+  if (dS_sum) *dS_sum = hs.cum_inf ? -INFINITY : fx_to_double(mk128((long)hs.cum_hi, (unsigned long)hs.cum_lo));
+  if (n_moves) *n_moves = (int32_t)hs.moves;
+  return USAP_OK;
+}
+
+int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, double mean,
+                     double thr, int32_t check, usap_sweep* out) {
+  if (int rc = check_engine(e)) return rc;
+  if (batch < 1 || !out) return USAP_EINVAL;
+  const int32_t nb = (int32_t)(((int64_t)e->N + batch - 1) / batch);
+  init_sweep(e, S, mean, thr, check);
+  HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
+  int32_t rebuilds = 0;
+  if (int rc = run_sweep(e, batch, 0, nb, stream0, &rebuilds)) return rc;
+  const Sweep& hs = *e->h_sweep;
+  const int32_t last = hs.done - 1;  // the last executed batch
+  e->last_beg = last * batch;
+  e->last_end = std::min<int64_t>((int64_t)e->last_beg + batch, e->N);
+  out->batches = hs.done;
+  out->proposals = std::min<int64_t>((int64_t)hs.done * batch, e->N);
+  out->converged = hs.stop == 1;
+  out->moves = (int64_t)hs.moves;
+  out->S = hs.S;
+  out->dS_sum = hs.dS_last;
+  out->dS_hi = (int64_t)hs.cum_hi;
+  out->dS_lo = (uint64_t)hs.cum_lo;
+  out->dS_inf = hs.cum_inf ? -1 : 0;
+  out->rebuilds = rebuilds;
   return USAP_OK;
 }
 
 int usap_nodal_sum_fx(usap_engine* e, int64_t* hi, uint64_t* lo, int32_t* inf) {
   if (int rc = check_engine(e)) return rc;
-  const Status& s = *e->h_status;
-  if (hi) *hi = (int64_t)s.sum_hi;
-  if (lo) *lo = (uint64_t)s.sum_lo;
-  if (inf) *inf = s.ninf > 0 ? -1 : 0;
+  const Sweep& hs = *e->h_sweep;
+  if (hi) *hi = (int64_t)hs.cum_hi;
+  if (lo) *lo = (uint64_t)hs.cum_lo;
+  if (inf) *inf = hs.cum_inf ? -1 : 0;
   return USAP_OK;
 }
 

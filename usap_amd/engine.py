@@ -3,7 +3,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import KERNELS, UsapKernelProf, UsapParams, UsapPhase, UsapStats, check, lib, ptr
+from ._lib import (KERNELS, UsapKernelProf, UsapParams, UsapPhase, UsapStats, UsapSweep, check,
+                   lib, ptr)
 
 FRAC = 48
 INIT_SCC, INIT_IDENTITY = 0, 1
@@ -128,6 +129,15 @@ class Engine:
             out.update(s=s[:n], dS=[fx(a, b) for a, b in zip(h[:n], l[:n])], inf=fi[:n],
                        accept=ac[:n])
         return out
+
+    def nodal_sweep(self, batch, stream0, S, mean, thr, check):
+        """One nodal sweep with the update step on the device (usap_nodal_sweep)."""
+        out = UsapSweep()
+        self._ck(lib().usap_nodal_sweep(self.h, batch, stream0, S, mean, thr, int(check),
+                                        C.byref(out)))
+        d = out.as_dict()
+        d["dS_fx"] = fx(d["dS_hi"], d["dS_lo"])
+        return d
 
     def snapshot(self, slot, S=0.0):
         self._ck(lib().usap_snapshot(self.h, slot, S))
