@@ -94,6 +94,7 @@ typedef struct usap_stats {
   double ms_golden;         /* snapshot / restore */
   int32_t rebuilds;         /* full M rebuilds */
   int32_t error;            /* 0 or USAP_E* */
+  int64_t grows;            /* M rows moved to a larger slab between batches */
 } usap_stats;
 
 typedef struct usap_engine usap_engine;

@@ -72,14 +72,15 @@ def test_sweep_equals_batch_loop(usap, orc, mode, slack):
 
 @pytest.mark.parametrize("gname", ["c1", "weighted"])
 def test_partition_with_tight_slabs(usap, orc, gname):
-    """slab_slack=1 makes hashed rows overflow often: every overflow rebuilds
-    M mid-sweep and resumes; the partition must not change."""
+    """slab_slack=1 makes hashed rows fill up often: rows are moved to larger
+    slabs between batches (and M is rebuilt when the pool runs out); the
+    partition must not change."""
     g = usap.config_graph("c1") if gname == "c1" else random_graph(
         700, 9000, seed=5, weighted=True, self_loops=True, hubs=2, isolated=7)
     params = dict(seed=2, init=1, ds_mode=1)
     eng = usap.Engine(g, slab_slack=1, **params)
     lab, st = eng.partition()
-    assert st["rebuilds"] > st["phases"] + 1  # overflow rebuilds happened
+    assert st["grows"] > 0  # rows were grown
     o = orc.Oracle(g, threads=8, **params)
     o.set_labels(np.arange(g.N, dtype=np.int32), g.N)
     B, olab, ost = o.partition()

@@ -78,6 +78,7 @@ class UsapStats(C.Structure):
         ("ms_golden", C.c_double),
         ("rebuilds", C.c_int32),
         ("error", C.c_int32),
+        ("grows", C.c_int64),
     ]
 
     def as_dict(self):

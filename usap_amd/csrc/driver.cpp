@@ -103,6 +103,7 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
   const int32_t N = g->N;
   usap_stats st{};
   const int rebuilds0 = usap::engine_rebuilds(e);
+  const int64_t grows0 = usap::engine_grows(e);
   const auto t_all = clk::now();
   int rc = USAP_OK;
 
@@ -245,6 +246,7 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
   st.S = S;
   st.B = B;
   st.rebuilds = usap::engine_rebuilds(e) - rebuilds0;
+  st.grows = usap::engine_grows(e) - grows0;
   st.error = rc;
   if (stats) *stats = st;
   if (rc) return rc;

@@ -116,8 +116,20 @@ struct MView {
   int32_t* key;
   int32_t* val;
   int32_t* bs;     // [B][kNBK]: sum of the row's counts per key bucket
+  int32_t* gmark;  // [B]: row queued for growth
+  int32_t* gq;     // [B]: growth queue
+  struct GrowState* gst;
   int32_t B;
   int32_t bshift;  // bucket of key k = k >> bshift
+};
+
+// Rows to grow between two batches; slabs for grown rows come from the free
+// tail of the pool.
+struct GrowState {
+  int32_t n;                  // rows queued by the last k_apply
+  int32_t nospace;            // the pool ran out: the host rebuilds M
+  unsigned long long used;    // pool slots in use
+  unsigned long long grows;   // rows grown since the engine was created
 };
 
 __device__ __forceinline__ uint32_t hash32(int32_t b) {
@@ -169,7 +181,13 @@ __device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int3
       k = atomicCAS(&m.key[o + h], -1, b);
       if (k == -1) {
         const int32_t n = atomicAdd(&m.nkeys[a], 1) + 1;
-        if (4 * n > 3 * c) atomicOr(flags, kFlagLoad);
+        if (4 * n > 3 * c) {
+          if (!kBuckets) {
+            atomicOr(flags, kFlagLoad);
+          } else if (atomicCAS(&m.gmark[a], 0, 1) == 0) {  // queue the row once
+            m.gq[atomicAdd(&m.gst->n, 1)] = a;
+          }
+        }
         k = b;
       }
     }
@@ -542,6 +560,87 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   for (int i = lane; i < kNBK; i += kWave) m.bs[a * kNBK + i] = bk[i];
 }
 
+// Moves every queued row to a slab 4x larger (or a direct row once that
+// reaches B/8) at the free tail of the pool, dropping keys whose count fell
+// to 0.  Runs between two batches; bucket sums are unchanged.
+struct GrowArgs {
+  int64_t* off;
+  int32_t* cap;
+  int32_t* nkeys;
+  int32_t* key;
+  int32_t* val;
+  int32_t* gmark;
+  const int32_t* gq;
+  GrowState* gst;
+  int64_t P;
+  int32_t B;
+  Sweep* sweep;
+};
+
+__global__ __launch_bounds__(256) void k_grow(GrowArgs g) {
+  __shared__ long long s_base;
+  __shared__ int32_t s_cap, s_live;
+  const int32_t nq = __hip_atomic_load(&g.gst->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const int32_t a = g.gq[qi];
+    const int64_t o = g.off[a];
+    const int32_t c = g.cap[a];
+    if (threadIdx.x == 0) {
+      int64_t nc = 4 * (int64_t)c;
+      if (8 * nc >= g.B) nc = g.B;
+      const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
+      if (nc <= c || (int64_t)base + nc > g.P) {
+        atomicAdd(&g.gst->used, (unsigned long long)(-nc));  // give the slots back
+        s_cap = 0;
+        g.gst->nospace = 1;
+        atomicOr(&g.sweep->flags, kFlagFull);
+        if (g.sweep->stop == 0) g.sweep->stop = 2;
+      } else {
+        s_cap = (int32_t)nc;
+        s_base = (long long)base;
+      }
+      s_live = 0;
+    }
+    __syncthreads();
+    const int32_t nc = s_cap;
+    if (nc > 0) {
+      const int64_t nb = s_base;
+      for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) {
+        g.key[nb + i] = -1;
+        g.val[nb + i] = 0;
+      }
+      __threadfence_block();
+      __syncthreads();
+      const bool dold = c == g.B, dnew = nc == g.B;
+      const uint32_t mask = (uint32_t)nc - 1;
+      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+        const int32_t k = dold ? j : g.key[o + j];
+        const int32_t v = g.val[o + j];
+        if (k < 0 || v == 0) continue;
+        g.val[o + j] = 0;  // the old slab stays in the pool as zeros until a rebuild
+        atomicAdd(&s_live, 1);
+        if (dnew) {
+          g.val[nb + k] = v;
+          continue;
+        }
+        uint32_t h = hash32(k) & mask;
+        while (atomicCAS(&g.key[nb + h], -1, k) != -1) h = (h + 1) & mask;
+        g.val[nb + h] = v;
+      }
+      __threadfence_block();
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        g.off[a] = nb;
+        g.cap[a] = nc;
+        g.nkeys[a] = s_live;
+        g.gst->grows += 1;
+      }
+    }
+    if (threadIdx.x == 0) g.gmark[a] = 0;
+    __syncthreads();
+  }
+}
+
 __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
   const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < N) labels[x] = map[labels[x]];
@@ -571,10 +670,14 @@ __device__ void block_store128(i128 v, long* out) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_entropy(const int32_t* __restrict__ val, int64_t nslots,
+__global__ __launch_bounds__(256) void k_entropy(const int32_t* __restrict__ val,
+                                                 const GrowState* gst,
                                                  const int32_t* __restrict__ d_out,
                                                  const int32_t* __restrict__ d_in, int32_t B,
-                                                 const int64_t* __restrict__ lam, long* part) {
+                                                 const int64_t* __restrict__ lam, int64_t P,
+                                                 long* part) {
+  const int64_t used = (int64_t)gst->used;  // every live slab lies below
+  const int64_t nslots = used < P ? used : P;
   i128 acc = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
@@ -871,6 +974,10 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
 __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   __shared__ int32_t s_tab[kNodalWaves][3][kTab];
   const int w = threadIdx.x / kWave, lane = lane_id();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
+    a.M.gst->n = 0;
+    a.Mt.gst->n = 0;
+  }
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
   if (n64 >= a.end || a.sweep->stop) return;
@@ -992,7 +1099,7 @@ __device__ void sweep_tail(Sweep* sw, Status* st) {
     sw->stop = 3;
   else if (sw->check && -sw->mean < sw->thr * sw->S)
     sw->stop = 1;
-  else if (flags & (kFlagLoad | kFlagFull))
+  else if (flags & kFlagFull)
     sw->stop = 2;
   st->sum_lo = 0;
   st->sum_hi = 0;
@@ -1110,8 +1217,11 @@ struct DevSpM {
   int32_t* val = nullptr;
   int32_t* bs = nullptr;  // [bs_rows][kNBK]
   int32_t bs_rows = 0;
+  int32_t* gmark = nullptr;  // [N]
+  int32_t* gq = nullptr;     // [N]
+  GrowState* gst = nullptr;
   int64_t P = 0;      // allocated slots
-  int64_t total = 0;  // slots in use
+  int64_t total = 0;  // slots in use after the last rebuild
 };
 
 struct usap_engine {
@@ -1225,6 +1335,9 @@ MView view(const DevSpM& m, int32_t B) {
   v.key = m.key;
   v.val = m.val;
   v.bs = m.bs;
+  v.gmark = m.gmark;
+  v.gq = m.gq;
+  v.gst = m.gst;
   v.B = B;
   v.bshift = bucket_shift(B);
   return v;
@@ -1312,10 +1425,10 @@ int rebuild_M(usap_engine* e) {
   for (int i = 0; i < 2; ++i) {
     DevSpM& m = *ms[i];
     m.total = tot[i];
-    if (m.total > m.P) {
+    if (2 * m.total > m.P) {  // half the pool is headroom for rows that grow
       dfree(m.key);
       dfree(m.val);
-      const int64_t P = m.total + m.total / 4 + 1024;
+      const int64_t P = 2 * m.total + (int64_t)4 * B + 4096;
       if (int rc = dalloc(e, &m.key, (size_t)P)) return rc;
       if (int rc = dalloc(e, &m.val, (size_t)P)) return rc;
       m.P = P;
@@ -1323,6 +1436,15 @@ int rebuild_M(usap_engine* e) {
     if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
         m.key, m.val, m.total);
     HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
+    HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
+    GrowState gs0{};
+    HIPCHK(hipMemcpyAsync(&gs0, m.gst, sizeof(GrowState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    gs0.n = 0;
+    gs0.nospace = 0;
+    gs0.used = (unsigned long long)m.total;
+    HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     if (m.bs_rows < B) {
       dfree(m.bs);
       if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
@@ -1366,6 +1488,15 @@ std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log;
 const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
 int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
+int64_t engine_grows(usap_engine* e) {
+  int64_t t = 0;
+  for (DevSpM* m : {&e->M, &e->Mt}) {
+    GrowState g{};
+    if (hipMemcpy(&g, m->gst, sizeof(GrowState), hipMemcpyDeviceToHost) == hipSuccess)
+      t += (int64_t)g.grows;
+  }
+  return t;
+}
 void engine_set_error(usap_engine* e, const char* msg) { e->err = msg; }
 }  // namespace usap
 
@@ -1415,6 +1546,9 @@ int usap_destroy(usap_engine* e) {
     dfree(m->key);
     dfree(m->val);
     dfree(m->bs);
+    dfree(m->gmark);
+    dfree(m->gq);
+    dfree(m->gst);
   }
   dfree(e->tmpM);
   dfree(e->tmpT);
@@ -1553,6 +1687,9 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
       TRY(dalloc(e, &m->off, (size_t)N + 1));
       TRY(dalloc(e, &m->cap, N));
       TRY(dalloc(e, &m->nkeys, N));
+      TRY(dalloc(e, &m->gmark, N));
+      TRY(dalloc(e, &m->gq, N));
+      TRY(dalloc(e, &m->gst, 1));
     }
     TRY(dalloc(e, &e->tmpM, (size_t)N + 1));
     TRY(dalloc(e, &e->tmpT, (size_t)N + 1));
@@ -1665,15 +1802,19 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
   if (int rc = check_engine(e)) return rc;
   if (!nnz) return USAP_EINVAL;
   const int32_t B = e->B;
+  hipStream_t st = e->stream;
+  GrowState gs{};
+  HIPCHK(hipMemcpyAsync(&gs, e->M.gst, sizeof(GrowState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t used = std::min<int64_t>((int64_t)gs.used, e->M.P);
   std::vector<int64_t> off((size_t)B + 1);
   std::vector<int32_t> capv((size_t)B);
-  std::vector<int32_t> key((size_t)e->M.total), val((size_t)e->M.total);
-  hipStream_t st = e->stream;
+  std::vector<int32_t> key((size_t)used), val((size_t)used);
   HIPCHK(hipMemcpyAsync(off.data(), e->M.off, sizeof(int64_t) * (B + 1), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(capv.data(), e->M.cap, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
-  if (e->M.total > 0) {
-    HIPCHK(hipMemcpyAsync(key.data(), e->M.key, sizeof(int32_t) * e->M.total, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(val.data(), e->M.val, sizeof(int32_t) * e->M.total, hipMemcpyDeviceToHost, st));
+  if (used > 0) {
+    HIPCHK(hipMemcpyAsync(key.data(), e->M.key, sizeof(int32_t) * used, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(val.data(), e->M.val, sizeof(int32_t) * used, hipMemcpyDeviceToHost, st));
   }
   HIPCHK(hipStreamSynchronize(st));
   std::vector<std::pair<int32_t, int32_t>> row;
@@ -1703,15 +1844,14 @@ int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
   if (int rc = check_engine(e)) return rc;
   hipStream_t st = e->stream;
   ev_rec(e, 0);
-  k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.val, e->M.total, e->d_out, e->d_in, e->B, e->lam,
-                                           e->ent_part);
+  k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.val, e->M.gst, e->d_out, e->d_in, e->B, e->lam,
+                                           e->M.P, e->ent_part);
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
   HIPCHK(hipMemcpyAsync(e->h_ent_part, e->ent_part, sizeof(long) * 2 * kEntropyBlocks,
                         hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  prof_add(e, USAP_K_ENTROPY, ev_ms(e, 0, 1), 1,
-           4.0 * (double)e->M.total + 8.0 * e->B + 16.0 * kEntropyBlocks);
+  prof_add(e, USAP_K_ENTROPY, ev_ms(e, 0, 1), 1, 8.0 * e->B + 16.0 * kEntropyBlocks);
   i128 t = 0;
   for (int i = 0; i < kEntropyBlocks; ++i)
     t += mk128(e->h_ent_part[2 * i], (unsigned long)e->h_ent_part[2 * i + 1]);
@@ -1902,6 +2042,21 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
   if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 1], st));
   k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
+  for (DevSpM* m : {&e->M, &e->Mt}) {
+    GrowArgs ga;
+    ga.off = m->off;
+    ga.cap = m->cap;
+    ga.nkeys = m->nkeys;
+    ga.key = m->key;
+    ga.val = m->val;
+    ga.gmark = m->gmark;
+    ga.gq = m->gq;
+    ga.gst = m->gst;
+    ga.P = m->P;
+    ga.B = B;
+    ga.sweep = e->sweep;
+    k_grow<<<64, 256, 0, st>>>(ga);
+  }
   if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 2], st));
   HIPCHK(hipGetLastError());
   return USAP_OK;

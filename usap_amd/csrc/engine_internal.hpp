@@ -12,5 +12,7 @@ const usap_graph* engine_host_graph(const usap_engine* e);
 std::vector<usap_phase>& engine_phase_log(usap_engine* e);
 const usap_params* engine_params(const usap_engine* e);
 int32_t engine_rebuilds(const usap_engine* e);
+// Rows moved to a larger slab between batches since the engine was created.
+int64_t engine_grows(usap_engine* e);
 void engine_set_error(usap_engine* e, const char* msg);
 }  // namespace usap
