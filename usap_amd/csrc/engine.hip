@@ -560,10 +560,14 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   for (int i = lane; i < kNBK; i += kWave) m.bs[a * kNBK + i] = bk[i];
 }
 
-// Moves every queued row to a slab 4x larger (or a direct row once that
-// reaches B/8) at the free tail of the pool, dropping keys whose count fell
-// to 0.  Runs between two batches; bucket sums are unchanged.
-struct GrowArgs {
+// Rows queued by k_apply (key count past 3/4 of the slab), handled between
+// two batches, zero counts dropped: a row whose live keys fit half its slab
+// (and at most kCompact slots) is compacted in place through LDS; any other
+// row moves to a slab >= 4*live+16 (direct once that reaches B/8) at the free
+// tail of the pool.  Bucket sums are unchanged.  One launch serves M and M^T.
+constexpr int kCompact = 8192;
+
+struct GrowSide {
   int64_t* off;
   int32_t* cap;
   int32_t* nkeys;
@@ -573,33 +577,90 @@ struct GrowArgs {
   const int32_t* gq;
   GrowState* gst;
   int64_t P;
+};
+
+struct GrowArgs {
+  GrowSide side[2];
   int32_t B;
   Sweep* sweep;
 };
 
-__global__ __launch_bounds__(256) void k_grow(GrowArgs g) {
+__global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
+  __shared__ int32_t s_k[kCompact];
+  __shared__ int32_t s_v[kCompact];
   __shared__ long long s_base;
   __shared__ int32_t s_cap, s_live;
-  const int32_t nq = __hip_atomic_load(&g.gst->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
-    const int32_t a = g.gq[qi];
+  const int32_t B = ga.B;
+  const int32_t n0 = __hip_atomic_load(&ga.side[0].gst->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t n1 = __hip_atomic_load(&ga.side[1].gst->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int32_t qi = blockIdx.x; qi < n0 + n1; qi += gridDim.x) {
+    const GrowSide& g = qi < n0 ? ga.side[0] : ga.side[1];
+    const int32_t a = g.gq[qi < n0 ? qi : qi - n0];
     const int64_t o = g.off[a];
     const int32_t c = g.cap[a];
+    const bool dold = c == B;
+    if (threadIdx.x == 0) s_live = 0;
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+      const int32_t k = dold ? j : g.key[o + j];
+      if (k >= 0 && g.val[o + j] != 0) atomicAdd(&s_live, 1);
+    }
+    __syncthreads();
+    const int32_t live = s_live;
+    if (!dold && 2 * live <= c && c <= kCompact) {
+      // in place: gather the live entries, clear, reinsert
+      __syncthreads();
+      if (threadIdx.x == 0) s_live = 0;
+      __syncthreads();
+      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+        const int32_t k = g.key[o + j], v = g.val[o + j];
+        if (k >= 0 && v != 0) {
+          const int32_t i = atomicAdd(&s_live, 1);
+          s_k[i] = k;
+          s_v[i] = v;
+        }
+      }
+      __syncthreads();
+      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+        g.key[o + j] = -1;
+        g.val[o + j] = 0;
+      }
+      __threadfence_block();
+      __syncthreads();
+      const uint32_t mask = (uint32_t)c - 1;
+      for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
+        uint32_t h = hash32(s_k[i]) & mask;
+        while (atomicCAS(&g.key[o + h], -1, s_k[i]) != -1) h = (h + 1) & mask;
+        g.val[o + h] = s_v[i];
+      }
+      __threadfence_block();
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        g.nkeys[a] = live;
+        g.gmark[a] = 0;
+        g.gst->grows += 1;
+      }
+      __syncthreads();
+      continue;
+    }
     if (threadIdx.x == 0) {
-      int64_t nc = 4 * (int64_t)c;
-      if (8 * nc >= g.B) nc = g.B;
+      int64_t nc = 32;
+      while (nc < 4 * (int64_t)live + 16) nc <<= 1;
+      if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
+      if (8 * nc >= B) nc = B;
       const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
-      if (nc <= c || (int64_t)base + nc > g.P) {
+      if (dold || (int64_t)base + nc > g.P) {
         atomicAdd(&g.gst->used, (unsigned long long)(-nc));  // give the slots back
         s_cap = 0;
-        g.gst->nospace = 1;
-        atomicOr(&g.sweep->flags, kFlagFull);
-        if (g.sweep->stop == 0) g.sweep->stop = 2;
+        if (!dold) {
+          g.gst->nospace = 1;
+          atomicOr(&ga.sweep->flags, kFlagFull);
+          if (ga.sweep->stop == 0) ga.sweep->stop = 2;
+        }
       } else {
         s_cap = (int32_t)nc;
         s_base = (long long)base;
       }
-      s_live = 0;
     }
     __syncthreads();
     const int32_t nc = s_cap;
@@ -611,14 +672,13 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs g) {
       }
       __threadfence_block();
       __syncthreads();
-      const bool dold = c == g.B, dnew = nc == g.B;
+      const bool dnew = nc == B;
       const uint32_t mask = (uint32_t)nc - 1;
       for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        const int32_t k = dold ? j : g.key[o + j];
+        const int32_t k = g.key[o + j];
         const int32_t v = g.val[o + j];
         if (k < 0 || v == 0) continue;
         g.val[o + j] = 0;  // the old slab stays in the pool as zeros until a rebuild
-        atomicAdd(&s_live, 1);
         if (dnew) {
           g.val[nb + k] = v;
           continue;
@@ -632,7 +692,7 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs g) {
       if (threadIdx.x == 0) {
         g.off[a] = nb;
         g.cap[a] = nc;
-        g.nkeys[a] = s_live;
+        g.nkeys[a] = live;
         g.gst->grows += 1;
       }
     }
@@ -1112,9 +1172,13 @@ __device__ void sweep_tail(Sweep* sw, Status* st) {
 
 // Moves of the accepted nodes of one batch, then their labels.  Old labels of
 // nodes inside the batch come from prop_r (k_nodal stored them), so writing a
-// moved node's label cannot race with a neighbour's read.
+// moved node's label cannot race with a neighbour's read.  Edges to
+// neighbours that stay put are first summed per neighbour block in a
+// per-wavefront LDS table, so each (row, column) count gets one atomic per
+// node instead of one per edge.
 __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   __shared__ int s_last;
+  __shared__ int32_t s_tab[kNodalWaves][3][kTab];
   if (a.sweep->stop) return;  // the same for every block of this launch
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
@@ -1126,10 +1190,26 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
     int32_t* flags = &a.status->flags;
     const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
     const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
+    const bool table = (o1 - o0) + (i1 - i0) <= kTabMaxDeg;
+    int32_t cap = 64;
+    while (cap < 2 * ((o1 - o0) + (i1 - i0)) && cap < kTab) cap <<= 1;
+    int32_t* tk = s_tab[w][0];
+    int32_t* two = s_tab[w][1];
+    int32_t* twi = s_tab[w][2];
+    const uint32_t mask = (uint32_t)cap - 1;
+    if (table) {
+      for (int32_t i = lane; i < cap; i += kWave) {
+        tk[i] = -1;
+        two[i] = 0;
+        twi[i] = 0;
+      }
+      wave_fence();
+    }
     long wo_sum = 0, wi_sum = 0;
     for (int64_t e = o0 + lane; e < o1; e += kWave) {
       const int32_t v = a.g.out_idx[e];
       const int32_t ww = a.g.out_w ? a.g.out_w[e] : 1;
+      wo_sum += ww;
       int32_t bo, bn;
       if (v >= a.beg && v < a.end) {
         bo = a.prop_r[v - a.beg];
@@ -1137,11 +1217,14 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
       } else {
         bo = bn = a.labels[v];
       }
+      if (table && bo == bn) {
+        tab_add(tk, two, mask, bo, ww);
+        continue;
+      }
       m_add<true>(a.M, r, bo, -ww, flags);
       m_add<true>(a.Mt, bo, r, -ww, flags);
       m_add<true>(a.M, s, bn, ww, flags);
       m_add<true>(a.Mt, bn, s, ww, flags);
-      wo_sum += ww;
     }
     for (int64_t e = i0 + lane; e < i1; e += kWave) {
       const int32_t v = a.g.in_idx[e];
@@ -1154,10 +1237,34 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
       } else {
         b = a.labels[v];
       }
+      if (table) {
+        tab_add(tk, twi, mask, b, ww);
+        continue;
+      }
       m_add<true>(a.M, b, r, -ww, flags);
       m_add<true>(a.Mt, r, b, -ww, flags);
       m_add<true>(a.M, b, s, ww, flags);
       m_add<true>(a.Mt, s, b, ww, flags);
+    }
+    if (table) {
+      wave_fence();
+      for (int32_t i = lane; i < cap; i += kWave) {
+        const int32_t b = tk[i];
+        if (b < 0) continue;
+        const int32_t wo = two[i], wi = twi[i];
+        if (wo) {
+          m_add<true>(a.M, r, b, -wo, flags);
+          m_add<true>(a.Mt, b, r, -wo, flags);
+          m_add<true>(a.M, s, b, wo, flags);
+          m_add<true>(a.Mt, b, s, wo, flags);
+        }
+        if (wi) {
+          m_add<true>(a.M, b, r, -wi, flags);
+          m_add<true>(a.Mt, r, b, -wi, flags);
+          m_add<true>(a.M, b, s, wi, flags);
+          m_add<true>(a.Mt, s, b, wi, flags);
+        }
+      }
     }
     wo_sum = wave_sum(wo_sum);
     wi_sum = wave_sum(wi_sum);
@@ -2042,21 +2149,23 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
   if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 1], st));
   k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
-  for (DevSpM* m : {&e->M, &e->Mt}) {
-    GrowArgs ga;
-    ga.off = m->off;
-    ga.cap = m->cap;
-    ga.nkeys = m->nkeys;
-    ga.key = m->key;
-    ga.val = m->val;
-    ga.gmark = m->gmark;
-    ga.gq = m->gq;
-    ga.gst = m->gst;
-    ga.P = m->P;
-    ga.B = B;
-    ga.sweep = e->sweep;
-    k_grow<<<64, 256, 0, st>>>(ga);
+  GrowArgs ga;
+  DevSpM* ms[2] = {&e->M, &e->Mt};
+  for (int i = 0; i < 2; ++i) {
+    GrowSide& gs = ga.side[i];
+    gs.off = ms[i]->off;
+    gs.cap = ms[i]->cap;
+    gs.nkeys = ms[i]->nkeys;
+    gs.key = ms[i]->key;
+    gs.val = ms[i]->val;
+    gs.gmark = ms[i]->gmark;
+    gs.gq = ms[i]->gq;
+    gs.gst = ms[i]->gst;
+    gs.P = ms[i]->P;
   }
+  ga.B = B;
+  ga.sweep = e->sweep;
+  k_grow<<<128, 256, 0, st>>>(ga);
   if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 2], st));
   HIPCHK(hipGetLastError());
   return USAP_OK;
