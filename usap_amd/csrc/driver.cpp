@@ -230,6 +230,12 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
       slot_of[id] = -1;
     }
     st.ms_golden += ms_since(t0);
+    if (p.verbose) {
+      long long gd[4];
+      usap::engine_grow_debug(e, gd);
+      std::fprintf(stderr, "usap:   growth: grown %lld compacted %lld failed-inserts %lld pool-full %lld\n",
+                   gd[0], gd[1], gd[2], gd[3]);
+    }
     if (p.verbose)
       std::fprintf(stderr,
                    "usap: phase %lld B=%d S=%.6f bracket=[%lld,%lld,%lld] next B=%lld sweeps=%d "

@@ -118,6 +118,9 @@ struct MView {
   int32_t* bs;     // [B][kNBK]: sum of the row's counts per key bucket
   int32_t* gmark;  // [B]: row queued for growth
   int32_t* gq;     // [B]: growth queue
+  int32_t* gpend;  // [B]: increments of the row waiting in the overflow buffer
+  int32_t* ovf;    // [3][ovf_cap]: (row, key, count) increments that found no free slot
+  int32_t ovf_cap;
   struct GrowState* gst;
   int32_t B;
   int32_t bshift;  // bucket of key k = k >> bshift
@@ -128,8 +131,13 @@ struct MView {
 struct GrowState {
   int32_t n;                  // rows queued by the last k_apply
   int32_t nospace;            // the pool ran out: the host rebuilds M
+  int32_t ovf_n;              // entries in the overflow buffer
+  uint32_t arrive;            // blocks of the current k_grow that finished
   unsigned long long used;    // pool slots in use
-  unsigned long long grows;   // rows grown since the engine was created
+  unsigned long long grows;   // rows grown or compacted since the engine was created
+  unsigned long long full;    // insertions that found no free slot
+  unsigned long long nospace_events;
+  unsigned long long compactions;
 };
 
 __device__ __forceinline__ uint32_t hash32(int32_t b) {
@@ -196,6 +204,18 @@ __device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int3
       return;
     }
     h = (h + 1) & mask;
+  }
+  if (kBuckets && w > 0) {  // slab full: park the increment until k_grow has grown the row
+    atomicAdd(&m.gst->full, 1ull);
+    const int32_t i = atomicAdd(&m.gst->ovf_n, 1);
+    if (i < m.ovf_cap) {
+      m.ovf[i] = a;
+      m.ovf[m.ovf_cap + i] = b;
+      m.ovf[2 * m.ovf_cap + i] = w;
+      atomicAdd(&m.gpend[a], 1);
+      if (atomicCAS(&m.gmark[a], 0, 1) == 0) m.gq[atomicAdd(&m.gst->n, 1)] = a;
+      return;
+    }
   }
   atomicOr(flags, w < 0 ? kFlagMissing : kFlagFull);
 }
@@ -566,6 +586,7 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
 // row moves to a slab >= 4*live+16 (direct once that reaches B/8) at the free
 // tail of the pool.  Bucket sums are unchanged.  One launch serves M and M^T.
 constexpr int kCompact = 8192;
+constexpr int32_t kOvfCap = 1 << 22;  // parked increments per orientation
 
 struct GrowSide {
   int64_t* off;
@@ -575,6 +596,9 @@ struct GrowSide {
   int32_t* val;
   int32_t* gmark;
   const int32_t* gq;
+  int32_t* gpend;
+  const int32_t* ovf;
+  int32_t ovf_cap;
   GrowState* gst;
   int64_t P;
 };
@@ -607,7 +631,8 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     }
     __syncthreads();
     const int32_t live = s_live;
-    if (!dold && 2 * live <= c && c <= kCompact) {
+    const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
+    if (!dold && 2 * (live + pend) <= c && c <= kCompact) {
       // in place: gather the live entries, clear, reinsert
       __syncthreads();
       if (threadIdx.x == 0) s_live = 0;
@@ -638,14 +663,16 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       if (threadIdx.x == 0) {
         g.nkeys[a] = live;
         g.gmark[a] = 0;
+        g.gpend[a] = 0;
         g.gst->grows += 1;
+        g.gst->compactions += 1;
       }
       __syncthreads();
       continue;
     }
     if (threadIdx.x == 0) {
       int64_t nc = 32;
-      while (nc < 4 * (int64_t)live + 16) nc <<= 1;
+      while (nc < 4 * ((int64_t)live + pend) + 16) nc <<= 1;
       if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
       if (8 * nc >= B) nc = B;
       const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
@@ -654,6 +681,7 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
         s_cap = 0;
         if (!dold) {
           g.gst->nospace = 1;
+          g.gst->nospace_events += 1;
           atomicOr(&ga.sweep->flags, kFlagFull);
           if (ga.sweep->stop == 0) ga.sweep->stop = 2;
         }
@@ -696,9 +724,61 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
         g.gst->grows += 1;
       }
     }
-    if (threadIdx.x == 0) g.gmark[a] = 0;
+    if (threadIdx.x == 0) {
+      g.gmark[a] = 0;
+      g.gpend[a] = 0;
+    }
     __syncthreads();
   }
+  // the last block replays the parked increments into the grown rows
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(&ga.side[0].gst->arrive, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  for (int side = 0; side < 2; ++side) {
+    const GrowSide& g = ga.side[side];
+    const int32_t no = min(__hip_atomic_load(&g.gst->ovf_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           g.ovf_cap);
+    for (int32_t i = threadIdx.x; i < no; i += blockDim.x) {
+      const int32_t a = g.ovf[i], b = g.ovf[g.ovf_cap + i], w = g.ovf[2 * g.ovf_cap + i];
+      const int64_t o = g.off[a];
+      const int32_t c = g.cap[a];
+      if (c == B) {
+        atomicAdd(&g.val[o + b], w);
+        continue;
+      }
+      const uint32_t mask = (uint32_t)c - 1;
+      uint32_t h = hash32(b) & mask;
+      bool done = false;
+      for (int32_t probe = 0; probe < c && !done; ++probe) {
+        int32_t k = __hip_atomic_load(&g.key[o + h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k < 0) {
+          k = atomicCAS(&g.key[o + h], -1, b);
+          if (k == -1) {
+            atomicAdd(&g.nkeys[a], 1);
+            k = b;
+          }
+        }
+        if (k == b) {
+          atomicAdd(&g.val[o + h], w);
+          done = true;
+        }
+        h = (h + 1) & mask;
+      }
+      if (!done) {  // still no room (the pool ran out): rebuild M
+        atomicOr(&ga.sweep->flags, kFlagFull);
+        if (ga.sweep->stop == 0) ga.sweep->stop = 2;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) g.gst->ovf_n = 0;
+  }
+  if (threadIdx.x == 0) ga.side[0].gst->arrive = 0;
 }
 
 __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
@@ -1326,6 +1406,8 @@ struct DevSpM {
   int32_t bs_rows = 0;
   int32_t* gmark = nullptr;  // [N]
   int32_t* gq = nullptr;     // [N]
+  int32_t* gpend = nullptr;  // [N]
+  int32_t* ovf = nullptr;    // [3][kOvfCap]
   GrowState* gst = nullptr;
   int64_t P = 0;      // allocated slots
   int64_t total = 0;  // slots in use after the last rebuild
@@ -1444,6 +1526,9 @@ MView view(const DevSpM& m, int32_t B) {
   v.bs = m.bs;
   v.gmark = m.gmark;
   v.gq = m.gq;
+  v.gpend = m.gpend;
+  v.ovf = m.ovf;
+  v.ovf_cap = kOvfCap;
   v.gst = m.gst;
   v.B = B;
   v.bshift = bucket_shift(B);
@@ -1544,11 +1629,14 @@ int rebuild_M(usap_engine* e) {
         m.key, m.val, m.total);
     HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
     HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
+    HIPCHK(hipMemsetAsync(m.gpend, 0, sizeof(int32_t) * B, st));
     GrowState gs0{};
     HIPCHK(hipMemcpyAsync(&gs0, m.gst, sizeof(GrowState), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     gs0.n = 0;
     gs0.nospace = 0;
+    gs0.ovf_n = 0;
+    gs0.arrive = 0;
     gs0.used = (unsigned long long)m.total;
     HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1595,6 +1683,17 @@ std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log;
 const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
 int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
+void engine_grow_debug(usap_engine* e, long long out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0;
+  for (DevSpM* m : {&e->M, &e->Mt}) {
+    GrowState g{};
+    if (hipMemcpy(&g, m->gst, sizeof(GrowState), hipMemcpyDeviceToHost) != hipSuccess) continue;
+    out[0] += (long long)g.grows;
+    out[1] += (long long)g.compactions;
+    out[2] += (long long)g.full;
+    out[3] += (long long)g.nospace_events;
+  }
+}
 int64_t engine_grows(usap_engine* e) {
   int64_t t = 0;
   for (DevSpM* m : {&e->M, &e->Mt}) {
@@ -1655,6 +1754,8 @@ int usap_destroy(usap_engine* e) {
     dfree(m->bs);
     dfree(m->gmark);
     dfree(m->gq);
+    dfree(m->gpend);
+    dfree(m->ovf);
     dfree(m->gst);
   }
   dfree(e->tmpM);
@@ -1796,6 +1897,8 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
       TRY(dalloc(e, &m->nkeys, N));
       TRY(dalloc(e, &m->gmark, N));
       TRY(dalloc(e, &m->gq, N));
+      TRY(dalloc(e, &m->gpend, N));
+      TRY(dalloc(e, &m->ovf, (size_t)3 * kOvfCap));
       TRY(dalloc(e, &m->gst, 1));
     }
     TRY(dalloc(e, &e->tmpM, (size_t)N + 1));
@@ -2160,6 +2263,9 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
     gs.val = ms[i]->val;
     gs.gmark = ms[i]->gmark;
     gs.gq = ms[i]->gq;
+    gs.gpend = ms[i]->gpend;
+    gs.ovf = ms[i]->ovf;
+    gs.ovf_cap = kOvfCap;
     gs.gst = ms[i]->gst;
     gs.P = ms[i]->P;
   }

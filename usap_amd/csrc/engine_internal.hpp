@@ -14,5 +14,8 @@ const usap_params* engine_params(const usap_engine* e);
 int32_t engine_rebuilds(const usap_engine* e);
 // Rows moved to a larger slab between batches since the engine was created.
 int64_t engine_grows(usap_engine* e);
+// Cumulative growth counters: rows grown or compacted, compactions, failed
+// insertions, pool exhaustions.
+void engine_grow_debug(usap_engine* e, long long out[4]);
 void engine_set_error(usap_engine* e, const char* msg);
 }  // namespace usap
