@@ -99,6 +99,82 @@ def cpu_baseline(g, params, phase_log, final_labels, final_B, seconds):
     }
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size: under `rocprofv3 --pmc FETCH_SIZE`
+    (and, in a second run, WRITE_SIZE) each of its dispatches should read
+    (write) nbytes; the ratio scales the same counters of the engine's
+    kernels to bytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {reps} copies of {nbytes} bytes")
+
+
+def measured_traffic(config):
+    """HBM bytes per k_nodal launch for this workload, as recorded in the
+    counter summary under profiles/ (None when there is none)."""
+    path = os.path.join(ROOT, "profiles", f"k_nodal_{config}_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def batched_bench(n_graphs=256, workers=0):
+    """The batched workload (n_graphs seeded DCSBMs, N log-uniform in
+    [500, 50000]) partitioned one graph at a time (the plain form: one
+    engine, its launches back to back) and then with several graphs in
+    flight (workers from the autotuner unless given).  Both runs must give
+    every graph the same labels and MDL."""
+    import usap_amd
+    t = time.perf_counter()
+    graphs, specs = usap_amd.batch_graphs(n=n_graphs, seed=4)
+    log(f"[bench] batched workload: {len(graphs)} graphs, {sum(g.N for g in graphs)} nodes, "
+        f"{sum(g.E for g in graphs)} edges, generated in {time.perf_counter() - t:.1f}s")
+    params = [dict(seed=s["seed"]) for s in specs]  # reference defaults: SCC init, ref dS
+    trials = []
+    if workers == 0:
+        t = time.perf_counter()
+        workers, trials = usap_amd.autotune(graphs, params)
+        log(f"[bench] autotune: {trials} -> {workers} workers ({time.perf_counter() - t:.1f}s)")
+    lab1, st1, b1 = usap_amd.partition_batch(graphs, params, workers=1)
+    log(f"[bench] plain: {b1['ms_total']:.0f} ms, {b1['graphs_per_s']:.2f} graphs/s")
+    labw, stw, bw = usap_amd.partition_batch(graphs, params, workers=workers)
+    log(f"[bench] batched ({workers} in flight): {bw['ms_total']:.0f} ms, "
+        f"{bw['graphs_per_s']:.2f} graphs/s")
+    same = (b1["failed"] == 0 and bw["failed"] == 0 and
+            all(np.array_equal(a, b) for a, b in zip(lab1, labw)) and
+            all(x["S"] == y["S"] for x, y in zip(st1, stw)))
+    return {
+        "metric": "graphs/s",
+        "value": round(bw["graphs_per_s"], 3),
+        "plain_graphs_per_s": round(b1["graphs_per_s"], 3),
+        "speedup_vs_plain": round(bw["graphs_per_s"] / max(b1["graphs_per_s"], 1e-12), 2),
+        "workers": workers,
+        "autotune": trials,
+        "same_mdl_and_labels": bool(same),
+        "graphs": len(graphs),
+        "nodes": int(sum(g.N for g in graphs)),
+        "edges": int(sum(g.E for g in graphs)),
+        "ms_plain": round(b1["ms_total"], 1),
+        "ms_batched": round(bw["ms_total"], 1),
+        "nodal_proposals": int(bw["nodal_proposals"]),
+        "mean_nmi": round(float(np.mean([usap_amd.nmi(g.truth, l)
+                                         for g, l in zip(graphs, labw)])), 4),
+        "workload": (f"{len(graphs)} directed DCSBMs, N log-uniform in [500, 50000], "
+                     "round(N^0.4) planted blocks, p_in uniform in [0.55, 0.85], out-degree "
+                     "d^-2.5 on [10,100], SCC init, K=10, nodal batch N/4"),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,11 +183,26 @@ def main():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graphs", type=int, default=256, help="batched workload size (c4)")
+    ap.add_argument("--workers", type=int, default=0, help="graphs in flight (c4; 0 = autotune)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calib:
+        copy_calibration()
+        return
+    if args.config == "c4":
+        out = batched_bench(args.graphs, args.workers)
+        out.update({"unit": "graphs/s", "n_gpus": 1, "higher_is_better": True,
+                    "scaling": "weak", "vs_baseline": None, "dtype": "int128",
+                    "data": "synthetic (seeded directed DCSBM generator)"})
+        print(json.dumps(out), flush=True)
+        return
 
     import usap_amd
     g, params, desc = workload(args.config)
@@ -148,7 +239,7 @@ def main():
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 5),
-        "traffic": None,
+        "traffic": measured_traffic(args.config),
         "bytes_per_launch": kp["bytes"] / max(kp["launches"], 1),
         "avg_launch_us": 1e3 * kp["ms"] / max(kp["launches"], 1),
         "launches": kp["launches"],

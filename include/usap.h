@@ -207,6 +207,38 @@ typedef struct usap_phase {
 /* Copies up to cap phase records of the last usap_partition; *n = count. */
 int usap_partition_log(usap_engine* e, int32_t cap, usap_phase* out, int32_t* n);
 
+/* ---- many graphs at once ---------------------------------------------------
+ * No counterpart in the reference, which partitions one graph per process
+ * (main.cpp:48-63) with partition() (graph_parallel.hpp:322-512) as the
+ * per-graph state machine.  For callers with many independent graphs. */
+
+typedef struct usap_batch_stats {
+  int32_t workers;         /* graphs in flight at once (one engine + HIP stream each) */
+  int32_t failed;          /* graphs whose partition failed (stats[i].error != 0) */
+  double ms_total;         /* wall time of the whole batch */
+  double graphs_per_s;
+  int64_t nodal_proposals; /* over all graphs (when stats is given) */
+} usap_batch_stats;
+
+/* Partitions graphs[0..n) (params[i] for graph i, as usap_create +
+ * usap_partition) with `workers` graphs in flight at once, each on its own
+ * engine and HIP stream, largest graphs first.  Graph i writes its N_i labels
+ * to labels + label_off[i] and its statistics to stats[i] (nullable).  Each
+ * graph's result is the one it gets alone.  workers = 0: chosen by
+ * usap_batch_autotune.  Returns USAP_ESTATE if any graph failed (see
+ * stats[i].error and out->failed). */
+int usap_partition_batch(int32_t n, const usap_graph* graphs, const usap_params* params,
+                         int32_t* labels, const int64_t* label_off, usap_stats* stats,
+                         int32_t workers, usap_batch_stats* out);
+
+/* Picks the number of graphs in flight: partitions a sample of `sample`
+ * graphs spread over the size range (0 = 24) with 2, 4, 8, ... workers (at
+ * most max_workers, 0 = 16) while graphs/s improves by 5% or more.  tried /
+ * rates (nullable, >= 16 entries) receive each trial's workers and graphs/s. */
+int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* params,
+                        int32_t max_workers, int32_t sample, int32_t* workers, int32_t* tried,
+                        double* rates);
+
 /* Kernel profile: HIP-event time on the engine's stream, launches, and the
  * algorithmic bytes the kernel moved (reads + writes the algorithm needs,
  * each counted once; 0 where not tallied). */

@@ -1,0 +1,74 @@
+"""Batched partitioning (usap_partition_batch): argument checks and the
+batched workload generator on the CPU; on the GPU, every graph of a batch
+gets exactly the partition it gets alone, whatever the number of graphs in
+flight."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import random_graph
+
+
+def test_batch_workload_generator(usap):
+    graphs, specs = usap.batch_graphs(n=16, seed=4, n_min=500, n_max=5000)
+    assert len(graphs) == 16
+    again, _ = usap.batch_graphs(n=16, seed=4, n_min=500, n_max=5000)
+    for g, h, s in zip(graphs, again, specs):
+        assert 500 <= g.N <= 5000 and g.N == s["N"]
+        assert s["K"] == max(2, round(s["N"] ** 0.4))
+        assert 0.55 <= s["p_in"] <= 0.85
+        np.testing.assert_array_equal(g.out_idx, h.out_idx)  # deterministic
+        assert len(np.unique(g.truth)) == s["K"]
+
+
+def test_batch_argument_checks(usap):
+    from usap_amd import _lib
+    L = _lib.lib()
+    bs = _lib.UsapBatchStats()
+    # an empty batch needs no device
+    assert L.usap_partition_batch(0, None, None, None, None, None, 1, C.byref(bs)) == 0
+    assert bs.failed == 0 and bs.graphs_per_s == 0.0
+    g = usap.config_graph("c1")
+    gs = (_lib.UsapGraph * 1)(g.as_struct())
+    ps = (_lib.UsapParams * 1)(usap.default_params())
+    lab = np.zeros(g.N, np.int32)
+    off = np.zeros(2, np.int64)
+    assert L.usap_partition_batch(1, gs, ps, lab.ctypes.data, off.ctypes.data, None, -1,
+                                  None) == _lib.USAP_EINVAL
+    assert L.usap_partition_batch(1, gs, ps, None, off.ctypes.data, None, 1,
+                                  None) == _lib.USAP_EINVAL
+    w = C.c_int32()
+    assert L.usap_batch_autotune(1, gs, ps, -1, 0, C.byref(w), None, None) == _lib.USAP_EINVAL
+    assert L.usap_batch_autotune(1, gs, ps, 4, 0, None, None, None) == _lib.USAP_EINVAL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("workers", [1, 3, 8])
+def test_batch_equals_alone(usap, workers):
+    graphs, specs = usap.batch_graphs(n=10, seed=7, n_min=300, n_max=4000)
+    graphs.append(random_graph(600, 7000, seed=3, weighted=True, self_loops=True, hubs=1,
+                               isolated=3))
+    params = [dict(seed=s["seed"], init=i % 2, ds_mode=(i // 2) % 2)
+              for i, s in enumerate(specs)] + [dict(seed=11, ds_mode=1)]
+    labs, sts, bs = usap.partition_batch(graphs, params, workers=workers)
+    assert bs["failed"] == 0 and bs["workers"] == min(workers, len(graphs))
+    assert bs["nodal_proposals"] == sum(s["nodal_proposals"] for s in sts)
+    for g, p, lb, sb in zip(graphs, params, labs, sts):
+        eng = usap.Engine(g, **p)
+        lab, st = eng.partition()
+        eng.close()
+        np.testing.assert_array_equal(lab, lb)
+        assert (st["S"], st["B"], st["sweeps"], st["batches"]) == \
+            (sb["S"], sb["B"], sb["sweeps"], sb["batches"])
+
+
+@pytest.mark.gpu
+def test_batch_autotune(usap):
+    graphs, specs = usap.batch_graphs(n=12, seed=9, n_min=300, n_max=3000)
+    params = [dict(seed=s["seed"]) for s in specs]
+    w, trials = usap.autotune(graphs, params, max_workers=8, sample=8)
+    assert 1 <= w <= 8 and trials and trials[0][0] == 2
+    assert all(r > 0 for _, r in trials)
+    labs, sts, bs = usap.partition_batch(graphs, params, workers=0)
+    assert bs["failed"] == 0 and 1 <= bs["workers"] <= 16

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libusap.so")
 
 USAP_OK = 0
+USAP_EINVAL, USAP_ENOMEM, USAP_EHIP, USAP_ESTATE, USAP_EINFEASIBLE, USAP_EIO = range(-1, -7, -1)
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "EINFEASIBLE", -6: "EIO"}
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -120,6 +121,19 @@ class UsapSweep(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class UsapBatchStats(C.Structure):
+    _fields_ = [
+        ("workers", C.c_int32),
+        ("failed", C.c_int32),
+        ("ms_total", C.c_double),
+        ("graphs_per_s", C.c_double),
+        ("nodal_proposals", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class UsapKernelProf(C.Structure):
     _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
 
@@ -155,6 +169,10 @@ SIGNATURES = [
     ("usap_partition", C.c_int, [_vp, _vp, C.POINTER(UsapStats)]),
     ("usap_partition_log", C.c_int, [_vp, C.c_int32, _vp, C.POINTER(C.c_int32)]),
     ("usap_profile", C.c_int, [_vp, C.c_int32, _vp]),
+    ("usap_partition_batch", C.c_int, [C.c_int32, _vp, _vp, _vp, _vp, _vp, C.c_int32,
+                                       C.POINTER(UsapBatchStats)]),
+    ("usap_batch_autotune", C.c_int, [C.c_int32, _vp, _vp, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_int32), _vp, _vp]),
     ("usap_golden_update", C.c_int, [_i64p, _f64p, _i64p, C.c_int64, C.c_double, C.c_int64,
                                      C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

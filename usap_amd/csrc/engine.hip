@@ -8,9 +8,13 @@
 //                   would need >= B slots (dense regime, small B);
 //   cap[a] <  B  -> open-addressing hash (power of two, linear probing),
 //                   keys int32 (-1 empty), counts int32.
-// Keys are never deleted inside a sweep (a count may drop to 0); a row whose
-// key count passes 3/4 of its slab raises an overflow flag and the host
-// rebuilds M from the labels after that batch.
+// Keys are never deleted inside a sweep (a count may drop to 0).  Each row
+// also keeps kNBK bucket sums (sum of its counts per key range) for the
+// two-level inverse-CDF pick.  A row whose key count passes 3/4 of its slab
+// is queued; an increment that finds no free slot is parked in an overflow
+// buffer; between two batches k_grow compacts queued rows in place or moves
+// them to a larger slab at the pool's tail and replays the parked increments.
+// The host rebuilds M from the labels only when the pool runs out.
 //
 // Kernels (one wavefront of 64 lanes per proposal item):
 //   k_degrees / k_caps / k_fill / k_insert_edges : M build from labels
@@ -24,7 +28,13 @@
 //        equals the reference's full rebuild (graph_parallel.hpp:438-443); its
 //        last block runs the sweep's update step and convergence test
 //        (graph_parallel.hpp:433-458) so a whole sweep runs without the host
+//   k_grow    : row growth / compaction and overflow replay between batches
 //   k_entropy : data_S over all non-zeros (graph_parallel.hpp:1281-1319)
+//
+// The nodal and apply kernels are latency bound (a few hundred bytes per node
+// at random rows of M); each wavefront keeps all loads of one step in flight
+// at once -- list ids, then labels, then rows, then the first probes of every
+// lookup -- instead of paying one memory latency per lookup.
 //
 // dS uses the factorised form of the reference's term sums: terms whose count
 // and degrees do not change cancel exactly in integer arithmetic, so a nodal
@@ -36,6 +46,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <rocprim/rocprim.hpp>
 #include <string>
@@ -70,14 +81,8 @@ enum : int32_t {
 };
 
 struct Status {
-  unsigned long long sum_lo;
-  unsigned long long sum_hi;
-  unsigned long long nodal_bytes;  // algorithmic bytes of k_nodal
-  unsigned long long apply_bytes;  // algorithmic bytes of k_apply
-  int32_t ninf;
-  int32_t moves;
-  int32_t flags;
-  int32_t pad;
+  int32_t flags;  // kFlag* raised by the running batch
+  int32_t pad[3];
 };
 
 // Device state of one nodal sweep (the reference's `update` task,
@@ -149,38 +154,81 @@ __device__ __forceinline__ int32_t ld_relaxed(int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// M[a][b]
-__device__ __forceinline__ int32_t m_get(const MView& m, int32_t a, int32_t b) {
-  const int64_t o = m.off[a];
-  const int32_t c = m.cap[a];
-  if (c == m.B) return m.val[o + b];
+// Slab of one row: offset and capacity.
+struct Row {
+  int64_t o;
+  int32_t c;
+};
+
+__device__ __forceinline__ Row m_row(const MView& m, int32_t a) { return Row{m.off[a], m.cap[a]}; }
+
+// First slot of key b in row R (the column itself in a direct row).
+__device__ __forceinline__ int64_t m_slot(const MView& m, Row R, int32_t b) {
+  return R.c == m.B ? R.o + b : R.o + (int64_t)(hash32(b) & (uint32_t)(R.c - 1));
+}
+
+// A lookup's first probe: the slot with the key and count found there.  The
+// callers issue several probes before resolving any, so a wavefront has all
+// of its lookups in flight at once instead of one memory latency per lookup.
+struct Probe {
+  int64_t slot;
+  int32_t k, v;
+};
+
+__device__ __forceinline__ Probe m_probe(const MView& m, Row R, int32_t b) {
+  Probe p;
+  p.slot = m_slot(m, R, b);
+  p.k = m.key[p.slot];
+  p.v = m.val[p.slot];
+  return p;
+}
+
+// Linear probing on from slot `slot` of a hashed row after a collision there.
+__device__ __forceinline__ int32_t m_value_more(const int32_t* key, const int32_t* val, int64_t o,
+                                             int32_t c, int32_t b, int64_t slot) {
   const uint32_t mask = (uint32_t)c - 1;
-  uint32_t h = hash32(b) & mask;
-  for (int32_t probe = 0; probe < c; ++probe) {
-    const int32_t k = m.key[o + h];
-    if (k == b) return m.val[o + h];
-    if (k < 0) return 0;
+  uint32_t h = (uint32_t)(slot - o);
+  for (int32_t probe = 1; probe < c; ++probe) {
     h = (h + 1) & mask;
+    const int32_t k = key[o + h], v = val[o + h];
+    if (k == b) return v;
+    if (k < 0) return 0;
   }
   return 0;
 }
 
-// M[a][b] += w (w < 0 only for a key that exists); kBuckets also updates the
-// row's bucket sum
+// M[a][b] from the first probe p of key b in row R of a.
+__device__ __forceinline__ int32_t m_value(const MView& m, Row R, int32_t b, const Probe& p) {
+  if (R.c == m.B || p.k == b) return p.v;
+  if (p.k < 0) return 0;
+  return m_value_more(m.key, m.val, R.o, R.c, b, p.slot);
+}
+
+// M[a][b]
+__device__ __forceinline__ int32_t m_get(const MView& m, int32_t a, int32_t b) {
+  const Row R = m_row(m, a);
+  return m_value(m, R, b, m_probe(m, R, b));
+}
+
+// M[a][b] += w in row R of a (w < 0 only for a key that exists), given the
+// key k read at the first probed slot `slot`.  Keys only go from empty to
+// set, so an earlier read is a valid first read: a stale "empty" makes the
+// claim fail and return the key that is there.  kBuckets also updates the
+// row's bucket sum and queues rows for growth instead of flagging a rebuild.
 template <bool kBuckets>
-__device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int32_t w,
-                                      int32_t* flags) {
+__device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32_t b, int32_t w,
+                                         int64_t slot, int32_t k, int32_t* flags) {
   if (kBuckets) atomicAdd(&m.bs[(int64_t)a * kNBK + (b >> m.bshift)], w);
-  const int64_t o = m.off[a];
-  const int32_t c = m.cap[a];
-  if (c == m.B) {
-    atomicAdd(&m.val[o + b], w);
+  if (R.c == m.B || k == b) {
+    atomicAdd(&m.val[slot], w);
     return;
   }
+  const int64_t o = R.o;
+  const int32_t c = R.c;
   const uint32_t mask = (uint32_t)c - 1;
-  uint32_t h = hash32(b) & mask;
+  uint32_t h = (uint32_t)(slot - o);
   for (int32_t probe = 0; probe < c; ++probe) {
-    int32_t k = ld_relaxed(&m.key[o + h]);
+    if (probe > 0) k = ld_relaxed(&m.key[o + h]);
     if (k < 0) {
       if (w < 0) {
         atomicOr(flags, kFlagMissing);
@@ -218,6 +266,22 @@ __device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int3
     }
   }
   atomicOr(flags, w < 0 ? kFlagMissing : kFlagFull);
+}
+
+// First key of b in row R, for m_add_at.
+__device__ __forceinline__ int32_t m_first_key(const MView& m, Row R, int32_t b, int64_t& slot) {
+  slot = m_slot(m, R, b);
+  return R.c == m.B ? b : ld_relaxed(&m.key[slot]);
+}
+
+// M[a][b] += w
+template <bool kBuckets>
+__device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int32_t w,
+                                      int32_t* flags) {
+  const Row R = m_row(m, a);
+  int64_t slot;
+  const int32_t k = m_first_key(m, R, b, slot);
+  m_add_at<kBuckets>(m, a, R, b, w, slot, k, flags);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -301,61 +365,108 @@ __device__ __forceinline__ long count_range_lane(const MView& m, int64_t o, int3
   return acc;
 }
 
-// scr[k - lo] += M[a][k] for lo <= k < hi (keys of one row are distinct, so
-// each lane owns the scratch entries it touches)
-__device__ __forceinline__ void add_range(const MView& m, int64_t o, int32_t c, int32_t lo,
-                                          int32_t hi, int32_t* scr) {
+// scr[k - lo] += M[a][k] for lo <= k < hi, from row R of a: a direct row is
+// read at columns [lo, hi), a hashed row's slab is scanned four slots per lane
+// at a time (all loads in flight before the LDS adds).
+__device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, int32_t hi,
+                                           int32_t* scr, long& bytes) {
   const int lane = lane_id();
-  if (c == m.B) {
-    for (int32_t i = lo + lane; i < hi; i += kWave) scr[i - lo] += m.val[o + i];
-  } else {
-    for (int32_t j = lane; j < c; j += kWave) {
-      const int32_t k = m.key[o + j];
-      if (k >= lo && k < hi) scr[k - lo] += m.val[o + j];
+  if (R.c == m.B) {
+    for (int32_t i = lane; i < hi - lo; i += kWave) atomicAdd(&scr[i], m.val[R.o + lo + i]);
+    bytes += 4 * (long)(hi - lo);
+    return;
+  }
+  for (int32_t base = 0; base < R.c; base += 4 * kWave) {
+    int32_t k[4], v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t j = base + q * kWave + lane;
+      k[q] = -1;
+      v[q] = 0;
+      if (j < R.c) {
+        k[q] = m.key[R.o + j];
+        v[q] = m.val[R.o + j];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k[q] >= lo && k[q] < hi && v[q] != 0) atomicAdd(&scr[k[q] - lo], v[q]);
+  }
+  bytes += 8 * (long)R.c;
+}
+
+// Index of the t-th unit in the LDS weights scr[0, width) (t < their sum):
+// each lane sums a contiguous run of entries, one wavefront scan finds the
+// run, and the lane owning it walks it.
+__device__ __forceinline__ int32_t scan_scratch(const int32_t* scr, int32_t width, int64_t t) {
+  const int lane = lane_id();
+  const int32_t per = (width + kWave - 1) / kWave;
+  const int32_t b0 = lane * per;
+  long sum = 0;
+  for (int32_t k = 0; k < per; ++k)
+    if (b0 + k < width) sum += scr[b0 + k];
+  const long inc = wave_incl_scan(sum);
+  const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
+  if (f < 0) return width - 1;
+  long tl = t - (__shfl(inc, f) - __shfl(sum, f));
+  int32_t res = width - 1;
+  if (lane == f) {
+    for (int32_t k = 0; k < per && b0 + k < width; ++k) {
+      const long w = scr[b0 + k];
+      if (tl < w) {
+        res = b0 + k;
+        break;
+      }
+      tl -= w;
     }
   }
+  return __shfl(res, f);
 }
 
 // Inverse-CDF pick over the combined weights M[u][i] + M[i][u] (row u of M
 // and row u of M^T), ascending i, skipping i == excl: the first i whose
 // running sum exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact
 // integer weights).  The caller guarantees t < total weight.  Two levels: the
-// rows' bucket sums locate the key bucket, then the bucket's keys (at most
-// 2^bshift of them) are summed in ascending order, in the wavefront's LDS
-// scratch `scr` when a row is hashed.
-__device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
-                                   int32_t excl, int32_t* scr, long& bytes) {
+// rows' bucket sums (all loaded at once) locate the key bucket [lo, hi); the
+// bucket's weights then come from one lookup per key when it is at most a
+// wavefront wide, else they are gathered into the wavefront's LDS scratch
+// `scr` (scr_cap ints) and searched with one scan.
+__device__ __forceinline__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
+                                   int32_t excl, int32_t* scr, int32_t scr_cap, long& bytes) {
   const int lane = lane_id();
   const int32_t B = M.B;
   const int sh = M.bshift;
-  const int64_t oM = M.off[u], oT = Mt.off[u];
-  const int32_t cM = M.cap[u], cT = Mt.cap[u];
+  const Row RM = m_row(M, u), RT = m_row(Mt, u);
   const int32_t nbk = ((B - 1) >> sh) + 1;
-  int32_t xb = -1;
-  long wx = 0;
-  if (excl >= 0) {
-    xb = excl >> sh;
-    wx = (long)m_get(M, u, excl) + (long)m_get(Mt, u, excl);
-    bytes += 16;
-  }
-  // level 1: bucket of the t-th unit
   const int32_t* bM = M.bs + (int64_t)u * kNBK;
   const int32_t* bT = Mt.bs + (int64_t)u * kNBK;
+  constexpr int kQ = kNBK / kWave;
+  long wq[kQ];
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    const int32_t j = q * kWave + lane;
+    wq[q] = j < nbk ? (long)bM[j] + (long)bT[j] : 0;
+  }
+  bytes += 24 + 8 * (long)nbk;
+  if (excl >= 0) {
+    const Probe p = m_probe(M, RM, excl), pt = m_probe(Mt, RT, excl);
+    const long wx = (long)m_value(M, RM, excl, p) + (long)m_value(Mt, RT, excl, pt);
+    bytes += 16;
+    const int32_t xb = excl >> sh;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+      if (q * kWave + lane == xb) wq[q] -= wx;
+  }
+  // level 1: bucket jb holding the t-th unit
   int32_t jb = nbk - 1;
-  for (int32_t base = 0; base < nbk; base += kWave) {
-    const int32_t j = base + lane;
-    long w = 0;
-    if (j < nbk) {
-      w = (long)bM[j] + (long)bT[j];
-      if (j == xb) w -= wx;
-    }
-    const long inc = wave_incl_scan(w);
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) {
+    const long inc = wave_incl_scan(wq[q]);
     const long tot = __shfl(inc, kWave - 1);
-    bytes += 8 * (long)min(kWave, nbk - base);
     if (t < tot) {
       const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
-      jb = base + f;
-      t -= __shfl(inc, f) - __shfl(w, f);
+      jb = q * kWave + f;
+      t -= __shfl(inc, f) - __shfl(wq[q], f);
       break;
     }
     t -= tot;
@@ -364,46 +475,51 @@ __device__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, i
   const int32_t lo = jb << sh;
   const int32_t hi = min(B, lo + (1 << sh));
   const int32_t width = hi - lo;
-  bytes += 24;
-  if (cM == B && cT == B) {
-    for (int32_t base = lo; base < hi; base += kWave) {
-      const int32_t i = base + lane;
-      long w = 0;
-      if (i < hi && i != excl) w = (long)M.val[oM + i] + (long)Mt.val[oT + i];
-      const long inc = wave_incl_scan(w);
-      const long tot = __shfl(inc, kWave - 1);
-      bytes += 8 * (long)min(kWave, hi - base);
-      if (t < tot) return base + (__ffsll((unsigned long long)__ballot(inc > t)) - 1);
-      t -= tot;
+  if (width <= kWave) {
+    long w = 0;
+    const int32_t i = lo + lane;
+    if (lane < width && i != excl) {
+      const Probe p = m_probe(M, RM, i), pt = m_probe(Mt, RT, i);
+      w = (long)m_value(M, RM, i, p) + (long)m_value(Mt, RT, i, pt);
     }
-    return hi - 1;
+    bytes += 16 * (long)width;
+    const long inc = wave_incl_scan(w);
+    const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
+    return f < 0 ? hi - 1 : lo + f;
   }
-  bytes += 4 * ((long)(cM == B ? width : cM) + (long)(cT == B ? width : cT));
-  if (width <= kScr) {
-    for (int32_t i = lane; i < width; i += kWave) scr[i] = 0;
-    wave_fence();
-    add_range(M, oM, cM, lo, hi, scr);
-    wave_fence();
-    add_range(Mt, oT, cT, lo, hi, scr);
+  if (width <= scr_cap) {
+    if (RM.c == B && RT.c == B) {
+      for (int32_t base = 0; base < width; base += 4 * kWave) {
+        int32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int32_t i = base + q * kWave + lane;
+          v[q] = i < width ? M.val[RM.o + lo + i] + Mt.val[RT.o + lo + i] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int32_t i = base + q * kWave + lane;
+          if (i < width) scr[i] = v[q];
+        }
+      }
+      bytes += 8 * (long)width;
+    } else {
+      for (int32_t i = lane; i < width; i += kWave) scr[i] = 0;
+      wave_fence();
+      gather_row(M, RM, lo, hi, scr, bytes);
+      gather_row(Mt, RT, lo, hi, scr, bytes);
+    }
     wave_fence();
     if (lane == 0 && excl >= lo && excl < hi) scr[excl - lo] = 0;
     wave_fence();
-    for (int32_t base = 0; base < width; base += kWave) {
-      const int32_t i = base + lane;
-      const long w = i < width ? (long)scr[i] : 0;
-      const long inc = wave_incl_scan(w);
-      const long tot = __shfl(inc, kWave - 1);
-      if (t < tot) return lo + base + (__ffsll((unsigned long long)__ballot(inc > t)) - 1);
-      t -= tot;
-    }
-    return hi - 1;
+    return lo + scan_scratch(scr, width, t);
   }
-  // buckets wider than the scratch (B > 786432): bisection inside the bucket
+  // buckets wider than the scratch: bisection inside the bucket
   int32_t a = lo, b = hi - 1;
   while (a < b) {
     const int32_t mid = a + (b - a) / 2;
-    const long c = wave_sum(count_range_lane(M, oM, cM, lo, mid, excl) +
-                            count_range_lane(Mt, oT, cT, lo, mid, excl));
+    const long c = wave_sum(count_range_lane(M, RM.o, RM.c, lo, mid, excl) +
+                            count_range_lane(Mt, RT.o, RT.c, lo, mid, excl));
     if (c > t)
       b = mid;
     else
@@ -879,14 +995,14 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
     s = (int32_t)usap::idx_of(x[2], B);
   } else {
     long bytes = 0;
-    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1, scr, bytes);
+    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1, scr, kScr, bytes);
     const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
     if (usap::uniform_branch(x[1], da, B)) {
       s = usap::uniform_not_r(x[2], r, B);
     } else {
       const int64_t W2 = da - m_get(a.M, u, r) - m_get(a.Mt, u, r);
       s = W2 == 0 ? usap::uniform_not_r(x[2], r, B)
-                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r, scr, bytes);
+                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r, scr, kScr, bytes);
     }
   }
   s_out = s;
@@ -962,6 +1078,20 @@ __global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
 // Nodal proposal, graph_parallel.hpp:990-1222
 // ---------------------------------------------------------------------------
 
+constexpr int kEdgeRegs = 4;     // list entries per lane per pass (256 per wavefront)
+constexpr int kScrNodal = 1024;  // k_nodal's LDS scratch per wavefront (one key bucket)
+
+// One workgroup's share of a batch, written by k_nodal / k_apply and summed
+// by the last block of k_apply (no same-address atomics per node).
+struct BlockPart {
+  long sum_hi;  // sum of the accepted finite dS of the block's nodes (int128)
+  unsigned long sum_lo;
+  unsigned long long nodal_bytes;  // algorithmic bytes of k_nodal
+  unsigned long long apply_bytes;  // algorithmic bytes of k_apply
+  int32_t moves;
+  int32_t ninf;
+};
+
 struct NodalArgs {
   GraphView g;
   MView M, Mt;
@@ -981,6 +1111,7 @@ struct NodalArgs {
   int32_t* prop_inf;
   int32_t* prop_acc;
   int32_t* prop_r;
+  BlockPart* part;
   Status* status;
   const Sweep* sweep;
 };
@@ -1002,63 +1133,132 @@ __device__ __forceinline__ void tab_add(int32_t* tk, int32_t* tv, uint32_t mask,
   }
 }
 
-// dS of moving node n from r to s (r != s), factorised.  tk/two/twi is a
-// table of `cap` slots (power of two) private to this wavefront.
-__device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* twi,
-                                            int32_t cap, const NodalArgs& a, int32_t n,
-                                            int32_t r, int32_t s, int64_t o0, int64_t kco,
-                                            int64_t i0, int64_t kci, int64_t k_out,
-                                            int64_t k_in, bool& inf, long& bytes) {
+// The neighbour-block table of node n: slots [0, cap) of tk (block), two
+// (weight of n's out-edges into it) and twi (in-edges), in LDS for nodes with
+// at most kTabMaxDeg list entries, else in the node's global-memory table.
+struct Table {
+  int32_t *tk, *two, *twi;
+  int32_t cap;
+};
+
+__device__ __forceinline__ Table tab_open(int32_t* lds, const int64_t* big_off, int32_t* big_tab,
+                                          int32_t n, int64_t kc) {
   const int lane = lane_id();
-  const uint32_t mask = (uint32_t)cap - 1;
+  Table T;
+  if (kc > kTabMaxDeg) {
+    int32_t cap = 1;
+    while (cap < 2 * kc) cap <<= 1;
+    T.cap = cap;
+    T.tk = big_tab + big_off[n];
+    T.two = T.tk + cap;
+    T.twi = T.tk + 2 * cap;
+  } else {
+    int32_t cap = kWave;  // >= 4/3 of the distinct neighbour blocks, <= kTab
+    while (cap < 2 * kc && cap < kTab) cap <<= 1;
+    T.cap = cap;
+    T.tk = lds;
+    T.two = lds + kTab;
+    T.twi = lds + 2 * kTab;
+  }
+  for (int32_t i = lane; i < T.cap; i += kWave) {
+    T.tk[i] = -1;
+    T.two[i] = 0;
+    T.twi[i] = 0;
+  }
+  wave_fence();
+  return T;
+}
+
+// Moves the table's entries, except those of blocks r and s (whose weights
+// are returned in wor/wir/wos/wis), to its front in place; returns their
+// number.  Chunk by chunk: every lane reads its slot before any write, and a
+// write never lands after the chunk being read.
+__device__ __forceinline__ int32_t tab_compact(const Table& T, int32_t r, int32_t s, long& wor,
+                                               long& wir, long& wos, long& wis) {
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int32_t nent = 0;
+  for (int32_t base = 0; base < T.cap; base += kWave) {  // cap is a multiple of kWave
+    const int32_t i = base + lane;
+    int32_t b = T.tk[i];
+    const int32_t wo = T.two[i], wi = T.twi[i];
+    const bool isr = b >= 0 && b == r, iss = b >= 0 && b == s;
+    wor += isr ? wo : 0;
+    wir += isr ? wi : 0;
+    wos += iss ? wo : 0;
+    wis += iss ? wi : 0;
+    if (isr || iss) b = -1;
+    const unsigned long long live = __ballot(b >= 0);
+    const int32_t pos = nent + __popcll(live & below);
+    wave_fence();
+    if (b >= 0) {
+      T.tk[pos] = b;
+      T.two[pos] = wo;
+      T.twi[pos] = wi;
+    }
+    nent += __popcll(live);
+    wave_fence();
+  }
+  return nent;
+}
+
+// List entry j of node n (out-edges then in-edges): neighbour id (-1 past the
+// end) and weight.
+__device__ __forceinline__ void list_entry(const GraphView& g, int64_t o0, int64_t kco,
+                                           int64_t i0, int64_t kc, int64_t j, int32_t& v,
+                                           int32_t& w) {
+  v = -1;
+  w = 1;
+  if (j < kco) {
+    v = g.out_idx[o0 + j];
+    if (g.out_w) w = g.out_w[o0 + j];
+  } else if (j < kc) {
+    v = g.in_idx[i0 + (j - kco)];
+    if (g.in_w) w = g.in_w[i0 + (j - kco)];
+  }
+}
+
+// dS of moving node n from r to s (r != s), factorised, from its complete
+// neighbour-block table.  All lookups of a pass are issued before any is
+// resolved.
+__device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, int32_t r,
+                                            int32_t s, Row rM, Row rT, int64_t dor, int64_t dir,
+                                            int64_t kc, int64_t k_out, int64_t k_in, long wself,
+                                            bool& inf, long& bytes) {
+  const int lane = lane_id();
   const int64_t* lam = a.lam;
-  for (int32_t i = lane; i < cap; i += kWave) {
-    tk[i] = -1;
-    two[i] = 0;
-    twi[i] = 0;
-  }
-  wave_fence();
-  // aggregate n's edges per distinct neighbour block
-  long wself = 0;
-  for (int64_t j = lane; j < kco; j += kWave) {
-    const int32_t v = a.g.out_idx[o0 + j];
-    const int32_t ww = a.g.out_w ? a.g.out_w[o0 + j] : 1;
-    if (v == n) wself += ww;
-    tab_add(tk, two, mask, a.labels[v], ww);
-  }
-  for (int64_t j = lane; j < kci; j += kWave) {
-    const int32_t v = a.g.in_idx[i0 + j];
-    const int32_t ww = a.g.in_w ? a.g.in_w[i0 + j] : 1;
-    tab_add(tk, twi, mask, a.labels[v], ww);
-  }
-  wave_fence();
+  // s's rows and degrees; the table is compacted while they load
+  const Row sM = m_row(a.M, s), sT = m_row(a.Mt, s);
+  const int64_t dos = a.d_out[s], dis = a.d_in[s];
+  long wor = 0, wir = 0, wos = 0, wis = 0;
+  const int32_t nent = tab_compact(T, r, s, wor, wir, wos, wis);
+  // corner: rows r, s at columns r, s
+  const Probe pRR = m_probe(a.M, rM, r), pRS = m_probe(a.M, rM, s);
+  const Probe pSR = m_probe(a.M, sM, r), pSS = m_probe(a.M, sM, s);
   // changed entries of rows r, s (out-edges) and columns r, s (in-edges)
   i128 acc = 0;
-  long Wo = 0, Wi = 0, wor = 0, wos = 0, wir = 0, wis = 0, lookups = 0;
+  long Wo = 0, Wi = 0, lookups = 0;
   bool neg = false;
-  for (int32_t i = lane; i < cap; i += kWave) {
-    const int32_t b = tk[i];
-    if (b < 0) continue;
-    const int32_t wo = two[i], wi = twi[i];
-    if (b == r) {
-      wor = wo;
-      wir = wi;
-      continue;
+  for (int32_t j = lane; j < nent; j += kWave) {
+    const int32_t b = T.tk[j], wo = T.two[j], wi = T.twi[j];
+    Probe p0{}, p1{}, p2{}, p3{};
+    if (wo > 0) {
+      p0 = m_probe(a.M, rM, b);
+      p1 = m_probe(a.M, sM, b);
     }
-    if (b == s) {
-      wos = wo;
-      wis = wi;
-      continue;
+    if (wi > 0) {
+      p2 = m_probe(a.Mt, rT, b);
+      p3 = m_probe(a.Mt, sT, b);
     }
     if (wo > 0) {
-      const int64_t Rr = m_get(a.M, r, b), Rs = m_get(a.M, s, b);
+      const int64_t Rr = m_value(a.M, rM, b, p0), Rs = m_value(a.M, sM, b, p1);
       lookups += 2;
       neg |= Rr - wo < 0;
       acc += Lterm(lam, Rr) + Lterm(lam, Rs) - Lterm(lam, Rr - wo) - Lterm(lam, Rs + wo);
       Wo += wo;
     }
     if (wi > 0) {
-      const int64_t Cr = m_get(a.Mt, r, b), Cs = m_get(a.Mt, s, b);
+      const int64_t Cr = m_value(a.Mt, rT, b, p2), Cs = m_value(a.Mt, sT, b, p3);
       lookups += 2;
       neg |= Cr - wi < 0;
       acc += Lterm(lam, Cr) + Lterm(lam, Cs) - Lterm(lam, Cr - wi) - Lterm(lam, Cs + wi);
@@ -1074,13 +1274,11 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
   wir = wave_sum(wir);
   wis = wave_sum(wis);
   wself = wave_sum(wself);
-  // neighbour ids + labels (+ weights), 2 lookups (key + count) per changed
-  // entry, 4 corner lookups, 4 degrees
-  bytes += (a.g.out_w ? 12 : 8) * (kco + kci) + 8 * (wave_sum(lookups) + 4) + 16;
-  // corner (rows r, s at columns r, s) and the degree factors
-  const int64_t Mrr = m_get(a.M, r, r), Mrs = m_get(a.M, r, s);
-  const int64_t Msr = m_get(a.M, s, r), Mss = m_get(a.M, s, s);
-  const int64_t dor = a.d_out[r], dos = a.d_out[s], dir = a.d_in[r], dis = a.d_in[s];
+  // neighbour ids + labels (+ weights); rows (offset + capacity) and degrees
+  // of r and s; 2 lookups (key + count) per changed entry and 4 corner lookups
+  bytes += (a.g.out_w ? 12 : 8) * kc + 48 + 16 + 8 * (wave_sum(lookups) + 4);
+  const int64_t Mrr = m_value(a.M, rM, r, pRR), Mrs = m_value(a.M, rM, s, pRS);
+  const int64_t Msr = m_value(a.M, sM, r, pSR), Mss = m_value(a.M, sM, s, pSS);
   const int64_t dor2 = dor - k_out, dos2 = dos + k_out, dir2 = dir - k_in, dis2 = dis + k_in;
   int64_t rr2, rs2, sr2, ss2;
   if (!a.exact) {  // graph_parallel.hpp:1117-1145: only out-edges update the rows
@@ -1111,65 +1309,95 @@ __device__ __forceinline__ i128 nodal_delta(int32_t* tk, int32_t* two, int32_t* 
   return old_c - new_c + acc - deg_old + deg_new;
 }
 
-__global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
-  __shared__ int32_t s_tab[kNodalWaves][3][kTab];
-  const int w = threadIdx.x / kWave, lane = lane_id();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
-    a.M.gst->n = 0;
-    a.Mt.gst->n = 0;
-  }
-  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
-  const int64_t n64 = (int64_t)a.beg + item;
-  if (n64 >= a.end || a.sweep->stop) return;
-  const int32_t n = (int32_t)n64;
+// One nodal proposal (graph_parallel.hpp:990-1222) for node n, batch item
+// `item`; returns its algorithmic bytes.  The loads form a short chain: own
+// label and list bounds; the list (ids) with r's rows and degrees; the
+// neighbours' labels with the drawn neighbour's; its block's degrees (and, when
+// the proposal follows M, its rows' bucket sums and one bucket); s's rows and
+// degrees; then every lookup of the dS at once.
+__device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int32_t n, int32_t* tab,
+                          int32_t* scr) {
+  const int lane = lane_id();
+  const GraphView& g = a.g;
   const int32_t B = a.B;
   const int32_t r = a.labels[n];
-  const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
-  const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
-  const int64_t kco = o1 - o0, kci = i1 - i0;
+  const int64_t o0 = g.out_ptr[n], i0 = g.in_ptr[n];
+  const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
+  const int64_t kc = kco + kci;
+  uint64_t x[4];
+  usap::philox4x64((uint64_t)n, 0, a.stream, usap::kTagNodal, a.seed, 0, x);
+  const Row rM = m_row(a.M, r), rT = m_row(a.Mt, r);
+  const int64_t dor = a.d_out[r], dir = a.d_in[r];
   int64_t k_out = kco, k_in = kci;
-  if (a.g.out_w) {
+  if (g.out_w) {
     long so = 0, si = 0;
-    for (int64_t j = lane; j < kco; j += kWave) so += a.g.out_w[o0 + j];
-    for (int64_t j = lane; j < kci; j += kWave) si += a.g.in_w[i0 + j];
+    for (int64_t j = lane; j < kco; j += kWave) so += g.out_w[o0 + j];
+    for (int64_t j = lane; j < kci; j += kWave) si += g.in_w[i0 + j];
     k_out = wave_sum(so);
     k_in = wave_sum(si);
   }
   const int64_t k = k_out + k_in;
-  // 4 CSR/CSC offsets, own label, outputs (s, dS, inf, accept)
-  long bytes = 32 + 4 + 28 + (a.g.out_w ? 4 * (kco + kci) : 0);
-  uint64_t x[4];
-  usap::philox4x64((uint64_t)n, 0, a.stream, usap::kTagNodal, a.seed, 0, x);
+  // list offsets, own label, outputs (s, dS, inf, accept, r) (+ weights)
+  long bytes = 32 + 4 + 32 + (g.out_w ? 4 * kc : 0);
+  const Table T = tab_open(tab, a.big_off, a.big_tab, n, kc);
+  const uint32_t mask = (uint32_t)T.cap - 1;
+  // the drawn neighbour (graph_parallel.hpp:1010-1027): unit t of the list
+  const int64_t t = k > 0 ? (int64_t)usap::idx_of(x[0], k) : 0;
+  int32_t vpick = -1;
+  if (k > 0) {
+    if (g.out_w)
+      vpick = pick_neighbor(g, o0, kco, i0, kci, t);
+    else if (t >= kEdgeRegs * kWave)
+      vpick = t < kco ? g.out_idx[o0 + t] : g.in_idx[i0 + (t - kco)];
+  }
+  int32_t u = -1;
+  int64_t da = 0;
+  long wself = 0;
+  for (int64_t base = 0; base < kc; base += kEdgeRegs * kWave) {
+    int32_t v[kEdgeRegs], ww[kEdgeRegs], lb[kEdgeRegs];
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q)
+      list_entry(g, o0, kco, i0, kc, base + q * kWave + lane, v[q], ww[q]);
+    if (base == 0 && k > 0 && vpick < 0) {  // unweighted, t < 256: lane t % 64 holds it
+      int32_t vv = v[0];
+#pragma unroll
+      for (int q = 1; q < kEdgeRegs; ++q)
+        if ((t >> 6) == q) vv = v[q];
+      vpick = __shfl(vv, (int)(t & (kWave - 1)));
+    }
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q) lb[q] = v[q] >= 0 ? a.labels[v[q]] : -1;
+    if (base == 0 && k > 0) {
+      u = a.labels[vpick];
+      da = (int64_t)a.d_out[u] + a.d_in[u];
+    }
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q) {
+      if (v[q] < 0) continue;
+      if (base + q * kWave + lane < kco) {
+        if (v[q] == n) wself += ww[q];
+        tab_add(T.tk, T.two, mask, lb[q], ww[q]);
+      } else {
+        tab_add(T.tk, T.twi, mask, lb[q], ww[q]);
+      }
+    }
+  }
   int32_t s;
   if (k == 0) {
     s = (int32_t)usap::idx_of(x[2], B);
   } else {
-    const int32_t v = pick_neighbor(a.g, o0, kco, i0, kci, usap::idx_of(x[0], k));
-    const int32_t u = a.labels[v];
-    const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
-    bytes += 16;  // neighbour id, its label, d_out[u], d_in[u]
+    bytes += 16;  // drawn neighbour id, its label, d_out[u], d_in[u]
     if (usap::uniform_branch(x[1], da, B))
       s = (int32_t)usap::idx_of(x[2], B);
     else
-      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1, &s_tab[w][0][0], bytes);
+      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1, scr, kScrNodal, bytes);
   }
   i128 dS = 0;
   bool inf = false;
   int32_t acc = 0;
   if (s != r) {
-    if (kco + kci > kTabMaxDeg) {
-      const int64_t bo = a.big_off[n];
-      int32_t cap = 1;
-      while (cap < 2 * (kco + kci)) cap <<= 1;
-      int32_t* base = a.big_tab + bo;
-      dS = nodal_delta(base, base + cap, base + 2 * cap, cap, a, n, r, s, o0, kco, i0, kci,
-                       k_out, k_in, inf, bytes);
-    } else {
-      int32_t cap = 64;  // >= 4/3 of the distinct neighbour blocks, <= kTab
-      while (cap < 2 * (kco + kci) && cap < kTab) cap <<= 1;
-      dS = nodal_delta(s_tab[w][0], s_tab[w][1], s_tab[w][2], cap, a, n, r, s, o0, kco, i0,
-                       kci, k_out, k_in, inf, bytes);
-    }
+    wave_fence();  // the table is complete
+    dS = nodal_delta(a, T, r, s, rM, rT, dor, dir, kc, k_out, k_in, wself, inf, bytes);
     acc = (inf || dS < 0) ? 1 : 0;
   }
   if (lane == 0) {
@@ -1179,7 +1407,30 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     a.prop_inf[item] = inf ? -1 : 0;
     a.prop_acc[item] = acc;
     a.prop_r[item] = r;
-    atomicAdd(&a.status->nodal_bytes, (unsigned long long)bytes + 4);
+  }
+  return bytes;
+}
+
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
+  __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
+  __shared__ __align__(8) int32_t s_scr[kNodalWaves][kScrNodal];
+  const int w = threadIdx.x / kWave;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
+    a.M.gst->n = 0;
+    a.Mt.gst->n = 0;
+  }
+  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
+  const int64_t n64 = (int64_t)a.beg + item;
+  long bytes = 0;
+  if (n64 < a.end && !a.sweep->stop) bytes = nodal_one(a, item, (int32_t)n64, s_tab[w], s_scr[w]);
+  __syncthreads();  // every wavefront is done with its scratch
+  unsigned long long* s_bytes = reinterpret_cast<unsigned long long*>(&s_scr[0][0]);
+  if (lane_id() == 0) s_bytes[w] = (unsigned long long)bytes;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kNodalWaves; ++i) t += s_bytes[i];
+    a.part[blockIdx.x].nodal_bytes = t;
   }
 }
 
@@ -1196,12 +1447,15 @@ struct ApplyArgs {
   int32_t* d_in;
   int32_t* bsize;
   int32_t beg, end;
+  const int64_t* big_off;
+  int32_t* big_tab;
   const int32_t* prop_s;
   const long* prop_hi;
   const unsigned long* prop_lo;
   const int32_t* prop_inf;
   const int32_t* prop_acc;
   const int32_t* prop_r;
+  BlockPart* part;
   Status* status;
   Sweep* sweep;
 };
@@ -1214,18 +1468,16 @@ __device__ double fx_to_double_dev(i128 v) {
   return neg ? -d : d;
 }
 
-// graph_parallel.hpp:433-458 after the moves of one batch: cumulative dS of
+// graph_parallel.hpp:433-458 after the moves of one batch (sum: exact sum of
+// the batch's accepted finite dS, ninf: accepted -inf ones): cumulative dS of
 // the sweep, S += dS_sum, the moving-average convergence test; clears the
-// batch status.  Same IEEE double operations as the host driver.
-__device__ void sweep_tail(Sweep* sw, Status* st) {
-  const unsigned long long lo = __hip_atomic_load(&st->sum_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long hi = __hip_atomic_load(&st->sum_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int32_t ninf = __hip_atomic_load(&st->ninf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int32_t moves = __hip_atomic_load(&st->moves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// batch flags.  Same IEEE double operations as the host driver.
+__device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_t moves,
+                           unsigned long long nbytes, unsigned long long abytes) {
   const int32_t flags = __hip_atomic_load(&st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  sw->nodal_bytes += __hip_atomic_load(&st->nodal_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  sw->apply_bytes += __hip_atomic_load(&st->apply_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const u128 cum = (((u128)sw->cum_hi << 64) | sw->cum_lo) + (((u128)hi << 64) | lo);
+  sw->nodal_bytes += nbytes;
+  sw->apply_bytes += abytes;
+  const u128 cum = (((u128)sw->cum_hi << 64) | sw->cum_lo) + sum;
   sw->cum_lo = (unsigned long long)cum;
   sw->cum_hi = (unsigned long long)(cum >> 64);
   if (ninf > 0) sw->cum_inf = 1;
@@ -1241,152 +1493,211 @@ __device__ void sweep_tail(Sweep* sw, Status* st) {
     sw->stop = 1;
   else if (flags & kFlagFull)
     sw->stop = 2;
-  st->sum_lo = 0;
-  st->sum_hi = 0;
-  st->nodal_bytes = 0;
-  st->apply_bytes = 0;
-  st->ninf = 0;
-  st->moves = 0;
   st->flags = 0;
 }
 
-// Moves of the accepted nodes of one batch, then their labels.  Old labels of
-// nodes inside the batch come from prop_r (k_nodal stored them), so writing a
-// moved node's label cannot race with a neighbour's read.  Edges to
-// neighbours that stay put are first summed per neighbour block in a
-// per-wavefront LDS table, so each (row, column) count gets one atomic per
-// node instead of one per edge.
-__global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
-  __shared__ int s_last;
-  __shared__ int32_t s_tab[kNodalWaves][3][kTab];
-  if (a.sweep->stop) return;  // the same for every block of this launch
-  const int w = threadIdx.x / kWave, lane = lane_id();
-  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
-  const int64_t n64 = (int64_t)a.beg + item;
-  if (n64 < a.end && a.prop_acc[item]) {
-    const int32_t n = (int32_t)n64;
-    const int32_t r = a.prop_r[item];
-    const int32_t s = a.prop_s[item];
-    int32_t* flags = &a.status->flags;
-    const int64_t o0 = a.g.out_ptr[n], o1 = a.g.out_ptr[n + 1];
-    const int64_t i0 = a.g.in_ptr[n], i1 = a.g.in_ptr[n + 1];
-    const bool table = (o1 - o0) + (i1 - i0) <= kTabMaxDeg;
-    int32_t cap = 64;
-    while (cap < 2 * ((o1 - o0) + (i1 - i0)) && cap < kTab) cap <<= 1;
-    int32_t* tk = s_tab[w][0];
-    int32_t* two = s_tab[w][1];
-    int32_t* twi = s_tab[w][2];
-    const uint32_t mask = (uint32_t)cap - 1;
-    if (table) {
-      for (int32_t i = lane; i < cap; i += kWave) {
-        tk[i] = -1;
-        two[i] = 0;
-        twi[i] = 0;
-      }
-      wave_fence();
-    }
-    long wo_sum = 0, wi_sum = 0;
-    for (int64_t e = o0 + lane; e < o1; e += kWave) {
-      const int32_t v = a.g.out_idx[e];
-      const int32_t ww = a.g.out_w ? a.g.out_w[e] : 1;
-      wo_sum += ww;
-      int32_t bo, bn;
-      if (v >= a.beg && v < a.end) {
-        bo = a.prop_r[v - a.beg];
-        bn = a.prop_acc[v - a.beg] ? a.prop_s[v - a.beg] : bo;
+// Moves accepted node n from block r to s: its edges' counts leave row and
+// column r for row and column s.  Edges to neighbours that stay put are first
+// summed per neighbour block in the node's table, so each changed count gets
+// one atomic per node; then the first keys of all eight changed entries of a
+// table entry are read together before the adds.  Old labels of nodes inside
+// the batch come from prop_r, so writing labels cannot race with their reads.
+// Returns the algorithmic bytes.
+__device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t r, int32_t s, int32_t* tab) {
+  const int lane = lane_id();
+  const GraphView& g = a.g;
+  int32_t* flags = &a.status->flags;
+  const int64_t o0 = g.out_ptr[n], i0 = g.in_ptr[n];
+  const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
+  const int64_t kc = kco + kci;
+  const Row rM = m_row(a.M, r), rT = m_row(a.Mt, r), sM = m_row(a.M, s), sT = m_row(a.Mt, s);
+  const Table T = tab_open(tab, a.big_off, a.big_tab, n, kc);
+  const uint32_t mask = (uint32_t)T.cap - 1;
+  long wo_sum = 0, wi_sum = 0;
+  for (int64_t base = 0; base < kc; base += kEdgeRegs * kWave) {
+    int32_t v[kEdgeRegs], ww[kEdgeRegs], bo[kEdgeRegs], bn[kEdgeRegs];
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q)
+      list_entry(g, o0, kco, i0, kc, base + q * kWave + lane, v[q], ww[q]);
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q) {
+      bo[q] = bn[q] = -1;
+      if (v[q] < 0) continue;
+      if (v[q] >= a.beg && v[q] < a.end) {
+        const int32_t x = v[q] - a.beg;
+        bo[q] = a.prop_r[x];
+        bn[q] = a.prop_acc[x] ? a.prop_s[x] : -2;
       } else {
-        bo = bn = a.labels[v];
+        bo[q] = a.labels[v[q]];
+        bn[q] = -2;
       }
-      if (table && bo == bn) {
-        tab_add(tk, two, mask, bo, ww);
-        continue;
-      }
-      m_add<true>(a.M, r, bo, -ww, flags);
-      m_add<true>(a.Mt, bo, r, -ww, flags);
-      m_add<true>(a.M, s, bn, ww, flags);
-      m_add<true>(a.Mt, bn, s, ww, flags);
     }
-    for (int64_t e = i0 + lane; e < i1; e += kWave) {
-      const int32_t v = a.g.in_idx[e];
-      const int32_t ww = a.g.in_w ? a.g.in_w[e] : 1;
-      wi_sum += ww;
-      int32_t b;
-      if (v >= a.beg && v < a.end) {
-        if (a.prop_acc[v - a.beg]) continue;  // moved as v's out-edge
-        b = a.prop_r[v - a.beg];
-      } else {
-        b = a.labels[v];
-      }
-      if (table) {
-        tab_add(tk, twi, mask, b, ww);
-        continue;
-      }
-      m_add<true>(a.M, b, r, -ww, flags);
-      m_add<true>(a.Mt, r, b, -ww, flags);
-      m_add<true>(a.M, b, s, ww, flags);
-      m_add<true>(a.Mt, s, b, ww, flags);
-    }
-    if (table) {
-      wave_fence();
-      for (int32_t i = lane; i < cap; i += kWave) {
-        const int32_t b = tk[i];
-        if (b < 0) continue;
-        const int32_t wo = two[i], wi = twi[i];
-        if (wo) {
-          m_add<true>(a.M, r, b, -wo, flags);
-          m_add<true>(a.Mt, b, r, -wo, flags);
-          m_add<true>(a.M, s, b, wo, flags);
-          m_add<true>(a.Mt, b, s, wo, flags);
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q) {
+      if (v[q] < 0) continue;
+      const int32_t w = ww[q];
+      if (base + q * kWave + lane < kco) {
+        wo_sum += w;
+        if (bn[q] == -2) {
+          tab_add(T.tk, T.two, mask, bo[q], w);
+        } else {  // the neighbour moved in this batch too: bo -> bn
+          m_add<true>(a.M, r, bo[q], -w, flags);
+          m_add<true>(a.Mt, bo[q], r, -w, flags);
+          m_add<true>(a.M, s, bn[q], w, flags);
+          m_add<true>(a.Mt, bn[q], s, w, flags);
         }
-        if (wi) {
-          m_add<true>(a.M, b, r, -wi, flags);
-          m_add<true>(a.Mt, r, b, -wi, flags);
-          m_add<true>(a.M, b, s, wi, flags);
-          m_add<true>(a.Mt, s, b, wi, flags);
-        }
-      }
-    }
-    wo_sum = wave_sum(wo_sum);
-    wi_sum = wave_sum(wi_sum);
-    if (lane == 0) {
-      atomicAdd(&a.status->apply_bytes,
-                (unsigned long long)(52 + 60 * ((o1 - o0) + (i1 - i0)) + 32 + 32 + 4));
-      if (wo_sum) {
-        atomicSub(&a.d_out[r], (int32_t)wo_sum);
-        atomicAdd(&a.d_out[s], (int32_t)wo_sum);
-      }
-      if (wi_sum) {
-        atomicSub(&a.d_in[r], (int32_t)wi_sum);
-        atomicAdd(&a.d_in[s], (int32_t)wi_sum);
-      }
-      atomicSub(&a.bsize[r], 1);
-      atomicAdd(&a.bsize[s], 1);
-      atomicAdd(&a.status->moves, 1);
-      if (a.prop_inf[item]) {
-        atomicAdd(&a.status->ninf, 1);
       } else {
-        // int128 add as two 64-bit atomics with an explicit carry: exact once
-        // every wave has added, in any order
-        const unsigned long lo = a.prop_lo[item];
-        const unsigned long long old = atomicAdd(&a.status->sum_lo, (unsigned long long)lo);
-        const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-        atomicAdd(&a.status->sum_hi, (unsigned long long)a.prop_hi[item] + carry);
+        wi_sum += w;
+        if (bn[q] == -2) tab_add(T.tk, T.twi, mask, bo[q], w);
+        // else: moved as the neighbour's out-edge
       }
-      a.labels[n] = s;
     }
   }
-  // the last block to finish runs the sweep step
+  wave_fence();
+  long x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+  const int32_t nent = tab_compact(T, -2, -2, x0, x1, x2, x3);
+  for (int32_t j = lane; j < nent; j += kWave) {
+    const int32_t b = T.tk[j], wo = T.two[j], wi = T.twi[j];
+    Row bM{0, 0}, bT{0, 0};
+    if (wi) bM = m_row(a.M, b);
+    if (wo) bT = m_row(a.Mt, b);
+    int64_t sl[8] = {};
+    int32_t k[8] = {};
+    if (wo) {
+      k[0] = m_first_key(a.M, rM, b, sl[0]);   // M[r][b]  -= wo
+      k[1] = m_first_key(a.Mt, bT, r, sl[1]);  // Mt[b][r] -= wo
+      k[2] = m_first_key(a.M, sM, b, sl[2]);   // M[s][b]  += wo
+      k[3] = m_first_key(a.Mt, bT, s, sl[3]);  // Mt[b][s] += wo
+    }
+    if (wi) {
+      k[4] = m_first_key(a.M, bM, r, sl[4]);   // M[b][r]  -= wi
+      k[5] = m_first_key(a.Mt, rT, b, sl[5]);  // Mt[r][b] -= wi
+      k[6] = m_first_key(a.M, bM, s, sl[6]);   // M[b][s]  += wi
+      k[7] = m_first_key(a.Mt, sT, b, sl[7]);  // Mt[s][b] += wi
+    }
+    if (wo) {
+      m_add_at<true>(a.M, r, rM, b, -wo, sl[0], k[0], flags);
+      m_add_at<true>(a.Mt, b, bT, r, -wo, sl[1], k[1], flags);
+      m_add_at<true>(a.M, s, sM, b, wo, sl[2], k[2], flags);
+      m_add_at<true>(a.Mt, b, bT, s, wo, sl[3], k[3], flags);
+    }
+    if (wi) {
+      m_add_at<true>(a.M, b, bM, r, -wi, sl[4], k[4], flags);
+      m_add_at<true>(a.Mt, r, rT, b, -wi, sl[5], k[5], flags);
+      m_add_at<true>(a.M, b, bM, s, wi, sl[6], k[6], flags);
+      m_add_at<true>(a.Mt, s, sT, b, wi, sl[7], k[7], flags);
+    }
+  }
+  wo_sum = wave_sum(wo_sum);
+  wi_sum = wave_sum(wi_sum);
+  if (lane == 0) {
+    if (wo_sum) {
+      atomicSub(&a.d_out[r], (int32_t)wo_sum);
+      atomicAdd(&a.d_out[s], (int32_t)wo_sum);
+    }
+    if (wi_sum) {
+      atomicSub(&a.d_in[r], (int32_t)wi_sum);
+      atomicAdd(&a.d_in[s], (int32_t)wi_sum);
+    }
+    atomicSub(&a.bsize[r], 1);
+    atomicAdd(&a.bsize[s], 1);
+    a.labels[n] = s;
+  }
+  return 52 + 60 * kc + 32 + 32 + 4;
+}
+
+// The batch's sums over all workgroups' shares, by the last block of k_apply,
+// then the sweep step.
+__device__ __forceinline__ void batch_tail(const ApplyArgs& a) {
+  __shared__ unsigned long s_lo[kNodalWaves];
+  __shared__ long s_hi[kNodalWaves];
+  __shared__ unsigned long long s_nb[kNodalWaves], s_ab[kNodalWaves];
+  __shared__ long s_mv[kNodalWaves], s_ni[kNodalWaves];
+  const int w = threadIdx.x / kWave;
+  i128 sum = 0;
+  long nb = 0, ab = 0, mv = 0, ni = 0;
+  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+    BlockPart* P = &a.part[i];
+    const unsigned long long lo = __hip_atomic_load((unsigned long long*)&P->sum_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long hi = __hip_atomic_load((unsigned long long*)&P->sum_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sum += (i128)(((u128)hi << 64) | (u128)lo);
+    nb += (long)__hip_atomic_load(&P->nodal_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ab += (long)__hip_atomic_load(&P->apply_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    mv += __hip_atomic_load(&P->moves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ni += __hip_atomic_load(&P->ninf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  sum = wave_sum128(sum);
+  nb = wave_sum(nb);
+  ab = wave_sum(ab);
+  mv = wave_sum(mv);
+  ni = wave_sum(ni);
+  if (lane_id() == 0) {
+    s_lo[w] = (unsigned long)(u128)sum;
+    s_hi[w] = (long)(sum >> 64);
+    s_nb[w] = (unsigned long long)nb;
+    s_ab[w] = (unsigned long long)ab;
+    s_mv[w] = mv;
+    s_ni[w] = ni;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
+    u128 tot = 0;
+    unsigned long long tnb = 0, tab = 0;
+    long tmv = 0, tni = 0;
+    for (int i = 0; i < kNodalWaves; ++i) {
+      tot += ((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i];
+      tnb += s_nb[i];
+      tab += s_ab[i];
+      tmv += s_mv[i];
+      tni += s_ni[i];
+    }
+    sweep_tail(a.sweep, a.status, tot, (int32_t)tni, (int32_t)tmv, tnb, tab);
+    a.sweep->arrive = 0;
+    __threadfence();
+  }
+}
+
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
+  __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
+  __shared__ unsigned long long s_ab[kNodalWaves];
+  __shared__ int s_last;
+  if (a.sweep->stop) return;  // the same for every block of this launch
+  const int w = threadIdx.x / kWave;
+  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
+  const int64_t n64 = (int64_t)a.beg + item;
+  long bytes = 0;
+  if (n64 < a.end && a.prop_acc[item])
+    bytes = apply_one(a, (int32_t)n64, a.prop_r[item], a.prop_s[item], s_tab[w]);
+  if (lane_id() == 0) s_ab[w] = (unsigned long long)bytes;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // this block's share of the batch sums
+    u128 sum = 0;
+    unsigned long long ab = 0;
+    int32_t moves = 0, ninf = 0;
+    for (int i = 0; i < kNodalWaves; ++i) {
+      const int64_t it = (int64_t)blockIdx.x * kNodalWaves + i;
+      ab += s_ab[i];
+      if (a.beg + it < a.end && a.prop_acc[it]) {
+        ++moves;
+        if (a.prop_inf[it])
+          ++ninf;
+        else
+          sum += ((u128)(unsigned long)a.prop_hi[it] << 64) | (u128)a.prop_lo[it];
+      }
+    }
+    BlockPart& P = a.part[blockIdx.x];
+    P.sum_lo = (unsigned long)sum;
+    P.sum_hi = (long)(sum >> 64);
+    P.apply_bytes = ab;
+    P.moves = moves;
+    P.ninf = ninf;
     __threadfence();
     s_last = atomicAdd(&a.sweep->arrive, 1u) == gridDim.x - 1;
   }
   __syncthreads();
-  if (s_last && threadIdx.x == 0) {
+  if (s_last) {
     __threadfence();
-    sweep_tail(a.sweep, a.status);
-    a.sweep->arrive = 0;
-    __threadfence();
+    batch_tail(a);
   }
 }
 
@@ -1454,6 +1765,7 @@ struct usap_engine {
   int32_t* prop_acc = nullptr;
   int32_t last_beg = 0, last_end = 0;
   Status* status = nullptr;   // the running batch
+  BlockPart* part = nullptr;  // per-workgroup shares of the running batch
   Sweep* sweep = nullptr;     // device sweep state
   Sweep* h_sweep = nullptr;   // pinned host copy
   int32_t* prop_r = nullptr;  // old label of each batch node
@@ -1490,22 +1802,63 @@ int set_err(usap_engine* e, int code, const std::string& msg) {
       return set_err(e, USAP_EHIP, std::string(#expr) + ": " + hipGetErrorString(_st)); \
   } while (0)
 
+// Device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the
+// engine's stream): creating and destroying engines never synchronises the
+// device, so many engines can come and go while others run (batched and
+// served graphs).  The default pool keeps freed memory for reuse.
+void prepare_pool(int device) {
+  static std::once_flag once;
+  std::call_once(once, [device]() {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+  });
+}
+
 template <typename T>
 int dalloc(usap_engine* e, T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
-  hipError_t st = hipMalloc((void**)p, n * sizeof(T));
+  hipError_t st = hipMallocAsync((void**)p, n * sizeof(T), e->stream);
   if (st != hipSuccess) {
     *p = nullptr;
-    return set_err(e, USAP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(st));
+    return set_err(e, USAP_ENOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(st));
   }
   return USAP_OK;
 }
 
 template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
+void dfree(usap_engine* e, T*& p) {
+  if (p) (void)hipFreeAsync((void*)p, e->stream);
   p = nullptr;
+}
+
+// Pinned host buffers are recycled through a process-wide free list instead
+// of being freed (hipHostFree may wait for the device).
+std::mutex g_pin_mu;
+std::vector<std::pair<size_t, void*>> g_pin_free;
+
+void* pin_get(size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    for (size_t i = 0; i < g_pin_free.size(); ++i)
+      if (g_pin_free[i].first == bytes) {
+        void* q = g_pin_free[i].second;
+        g_pin_free.erase(g_pin_free.begin() + (long)i);
+        return q;
+      }
+  }
+  void* q = nullptr;
+  if (hipHostMalloc(&q, bytes) != hipSuccess) return nullptr;
+  return q;
+}
+
+void pin_put(void* q, size_t bytes) {
+  if (!q) return;
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  g_pin_free.emplace_back(bytes, q);
 }
 
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
@@ -1599,8 +1952,8 @@ int rebuild_M(usap_engine* e) {
   HIPCHK(rocprim::exclusive_scan(nullptr, bytes, e->tmpM, e->M.off, (int64_t)0, (size_t)B + 1,
                                  rocprim::plus<int64_t>(), st));
   if (bytes > e->scan_tmp_bytes) {
-    dfree(e->scan_tmp);
-    HIPCHK(hipMalloc(&e->scan_tmp, bytes));
+    dfree(e, e->scan_tmp);
+    HIPCHK(hipMallocAsync(&e->scan_tmp, bytes, st));
     e->scan_tmp_bytes = bytes;
   }
   bytes = e->scan_tmp_bytes;
@@ -1618,8 +1971,8 @@ int rebuild_M(usap_engine* e) {
     DevSpM& m = *ms[i];
     m.total = tot[i];
     if (2 * m.total > m.P) {  // half the pool is headroom for rows that grow
-      dfree(m.key);
-      dfree(m.val);
+      dfree(e, m.key);
+      dfree(e, m.val);
       const int64_t P = 2 * m.total + (int64_t)4 * B + 4096;
       if (int rc = dalloc(e, &m.key, (size_t)P)) return rc;
       if (int rc = dalloc(e, &m.val, (size_t)P)) return rc;
@@ -1641,7 +1994,7 @@ int rebuild_M(usap_engine* e) {
     HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     if (m.bs_rows < B) {
-      dfree(m.bs);
+      dfree(e, m.bs);
       if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
       m.bs_rows = B;
     }
@@ -1731,59 +2084,63 @@ const char* usap_last_error(const usap_engine* e) { return e ? e->err.c_str() : 
 int usap_destroy(usap_engine* e) {
   if (!e) return USAP_OK;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
-  dfree(e->out_ptr);
-  dfree(e->in_ptr);
-  dfree(e->out_idx);
-  dfree(e->in_idx);
-  dfree(e->out_w);
-  dfree(e->in_w);
-  dfree(e->out_src);
-  dfree(e->lam);
-  dfree(e->labels);
-  dfree(e->d_out);
-  dfree(e->d_in);
-  dfree(e->ecnt_out);
-  dfree(e->ecnt_in);
-  dfree(e->bsize);
+  dfree(e, e->out_ptr);
+  dfree(e, e->in_ptr);
+  dfree(e, e->out_idx);
+  dfree(e, e->in_idx);
+  dfree(e, e->out_w);
+  dfree(e, e->in_w);
+  dfree(e, e->out_src);
+  dfree(e, e->lam);
+  dfree(e, e->labels);
+  dfree(e, e->d_out);
+  dfree(e, e->d_in);
+  dfree(e, e->ecnt_out);
+  dfree(e, e->ecnt_in);
+  dfree(e, e->bsize);
   for (DevSpM* m : {&e->M, &e->Mt}) {
-    dfree(m->off);
-    dfree(m->cap);
-    dfree(m->nkeys);
-    dfree(m->key);
-    dfree(m->val);
-    dfree(m->bs);
-    dfree(m->gmark);
-    dfree(m->gq);
-    dfree(m->gpend);
-    dfree(m->ovf);
-    dfree(m->gst);
+    dfree(e, m->off);
+    dfree(e, m->cap);
+    dfree(e, m->nkeys);
+    dfree(e, m->key);
+    dfree(e, m->val);
+    dfree(e, m->bs);
+    dfree(e, m->gmark);
+    dfree(e, m->gq);
+    dfree(e, m->gpend);
+    dfree(e, m->ovf);
+    dfree(e, m->gst);
   }
-  dfree(e->tmpM);
-  dfree(e->tmpT);
-  if (e->scan_tmp) (void)hipFree(e->scan_tmp);
-  dfree(e->best_s);
-  dfree(e->best_hi);
-  dfree(e->best_lo);
-  dfree(e->map);
-  dfree(e->prop_s);
-  dfree(e->prop_hi);
-  dfree(e->prop_lo);
-  dfree(e->prop_inf);
-  dfree(e->prop_acc);
-  dfree(e->status);
-  dfree(e->sweep);
-  dfree(e->prop_r);
-  dfree(e->build_flags);
+  dfree(e, e->tmpM);
+  dfree(e, e->tmpT);
+  dfree(e, e->scan_tmp);
+  dfree(e, e->best_s);
+  dfree(e, e->best_hi);
+  dfree(e, e->best_lo);
+  dfree(e, e->map);
+  dfree(e, e->prop_s);
+  dfree(e, e->prop_hi);
+  dfree(e, e->prop_lo);
+  dfree(e, e->prop_inf);
+  dfree(e, e->prop_acc);
+  dfree(e, e->part);
+  dfree(e, e->status);
+  dfree(e, e->sweep);
+  dfree(e, e->prop_r);
+  dfree(e, e->build_flags);
   for (hipEvent_t ev : e->bev) (void)hipEventDestroy(ev);
-  if (e->h_sweep) (void)hipHostFree(e->h_sweep);
-  dfree(e->big_off);
-  dfree(e->big_tab);
-  dfree(e->ent_part);
-  if (e->h_ent_part) (void)hipHostFree(e->h_ent_part);
-  for (int i = 0; i < kNumSnap; ++i) dfree(e->snap_labels[i]);
+  pin_put(e->h_sweep, sizeof(Sweep));
+  dfree(e, e->big_off);
+  dfree(e, e->big_tab);
+  dfree(e, e->ent_part);
+  pin_put(e->h_ent_part, sizeof(long) * 2 * kEntropyBlocks);
+  for (int i = 0; i < kNumSnap; ++i) dfree(e, e->snap_labels[i]);
   for (hipEvent_t& ev : e->ev)
     if (ev) (void)hipEventDestroy(ev);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->stream) {
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipStreamDestroy(e->stream);
+  }
   delete e;
   return USAP_OK;
 }
@@ -1855,6 +2212,7 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     usap_destroy(e);
     return rc;
   }
+  prepare_pool(e->p.device);
   hst = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
   if (hst != hipSuccess) {
     e->stream = nullptr;
@@ -1913,12 +2271,14 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     TRY(dalloc(e, &e->prop_inf, N));
     TRY(dalloc(e, &e->prop_acc, N));
     TRY(dalloc(e, &e->status, 1));
+    TRY(dalloc(e, &e->part, blocks_for(N, kNodalWaves)));
     TRY(dalloc(e, &e->sweep, 1));
     TRY(dalloc(e, &e->prop_r, N));
     TRY(dalloc(e, &e->build_flags, 1));
     TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
-    if (hipHostMalloc((void**)&e->h_sweep, sizeof(Sweep)) != hipSuccess ||
-        hipHostMalloc((void**)&e->h_ent_part, sizeof(long) * 2 * kEntropyBlocks) != hipSuccess) {
+    e->h_sweep = (Sweep*)pin_get(sizeof(Sweep));
+    e->h_ent_part = (long*)pin_get(sizeof(long) * 2 * kEntropyBlocks);
+    if (!e->h_sweep || !e->h_ent_part) {
       rc = set_err(e, USAP_ENOMEM, "hipHostMalloc");
       goto fail;
     }
@@ -2220,6 +2580,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   a.prop_inf = e->prop_inf;
   a.prop_acc = e->prop_acc;
   a.prop_r = e->prop_r;
+  a.part = e->part;
   a.status = e->status;
   a.sweep = e->sweep;
   ApplyArgs b;
@@ -2238,6 +2599,9 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   b.prop_inf = e->prop_inf;
   b.prop_acc = e->prop_acc;
   b.prop_r = e->prop_r;
+  b.big_off = e->big_off;
+  b.big_tab = e->big_tab;
+  b.part = e->part;
   b.status = e->status;
   b.sweep = e->sweep;
   const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
