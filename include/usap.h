@@ -109,6 +109,12 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out);
 int usap_destroy(usap_engine* e);
 const char* usap_last_error(const usap_engine* e);
 
+/* Replace the engine's edges (same N), keeping the labels and B; M and the
+ * block degrees are rebuilt for the new edges.  No counterpart in the
+ * reference (one graph per Graph_P object, graph_parallel.hpp:61-78); for
+ * callers that receive a graph as streamed edge increments. */
+int usap_set_graph(usap_engine* e, const usap_graph* g);
+
 /* Set labels (all in [0,B)) and rebuild M and the block degrees.
  * ≙ labels assignment + _initialize_edge_counts (graph_parallel.hpp:703-759). */
 int usap_set_labels(usap_engine* e, const int32_t* labels, int32_t B);
@@ -188,6 +194,15 @@ int usap_restore(usap_engine* e, int32_t slot);
  * labels_out (nullable, [N]); stats (nullable). */
 int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats);
 
+/* The nodal phase (graph_parallel.hpp:381-478) at the current labels and B:
+ * sweeps of batches from nodal stream stream0 until the convergence test
+ * with delta_entropy_threshold2 fires (or max_nodal_itr sweeps), then the
+ * MDL.  No merge phase and no golden section.  No counterpart in the
+ * reference; for callers that resume the chain on a grown graph
+ * (usap_set_graph) from the previous partition.  labels_out / stats
+ * nullable; stats->S and ->B are the final MDL and B. */
+int usap_resume(usap_engine* e, uint64_t stream0, int32_t* labels_out, usap_stats* stats);
+
 /* Per-phase record of the last usap_partition (one per merge phase). */
 typedef struct usap_phase {
   int32_t B;           /* blocks after the merge */
@@ -230,6 +245,71 @@ typedef struct usap_batch_stats {
 int usap_partition_batch(int32_t n, const usap_graph* graphs, const usap_params* params,
                          int32_t* labels, const int64_t* label_off, usap_stats* stats,
                          int32_t workers, usap_batch_stats* out);
+
+/* ---- serving graphs that arrive as streamed edge increments ---------------
+ * No counterpart in the reference (streaming is future work there,
+ * README.md:47-49).  For callers that serve partition requests over time. */
+
+typedef struct usap_serve_graph {
+  int32_t N;
+  int64_t E;
+  const int32_t* src;   /* [E] 0-based edge sources in delivery order */
+  const int32_t* dst;   /* [E] */
+  const int32_t* w;     /* [E] weights, NULL = unweighted */
+  double arrival_s;     /* first increment's arrival, seconds after the start */
+  usap_params params;
+} usap_serve_graph;
+
+typedef struct usap_serve_opts {
+  int32_t stages;       /* increments per graph: stage k holds the first ceil(E*k/stages) edges */
+  double interval_s;    /* time between a graph's increments */
+  int32_t workers;      /* stages in flight at once (one host thread each) */
+  int64_t hbm_budget;   /* device bytes for resident graphs by usap_footprint (0 = no limit) */
+} usap_serve_opts;
+
+typedef struct usap_serve_stage {  /* one per (graph, stage), graph-major */
+  int32_t graph, stage;
+  double t_ready;       /* increment arrived (s since the start) */
+  double t_start, t_done;
+  double t_admit;       /* stage 1 only: graph admitted */
+  double t_release;     /* stage 1 only: graph left the device */
+  double S;             /* MDL after this stage */
+  int32_t B;
+  int32_t error;
+  int64_t E;            /* edges in this stage */
+  int64_t sweeps;
+  int64_t nodal_proposals;
+} usap_serve_stage;
+
+typedef struct usap_serve_stats {
+  double wall_s;
+  double graphs_per_s;  /* graphs / (last completion - first arrival) */
+  double p50_ms, p99_ms;                /* stage latency t_done - t_ready, all stages */
+  double p50_first_ms, p99_first_ms;    /* stage 1 (full partition) */
+  double p50_resume_ms, p99_resume_ms;  /* stages 2.. (resumed chain) */
+  int32_t max_resident; /* graphs resident at once */
+  int32_t failed;
+  int64_t max_bytes;    /* their summed footprint at the peak */
+} usap_serve_stats;
+
+/* Estimated device bytes of an engine for a graph of N nodes and E edges
+ * (graph arrays, state, bucket sums and M slabs at B = N), used for
+ * admission. */
+int64_t usap_footprint(int32_t N, int64_t E);
+
+/* Serves n graphs: graph g arrives at graphs[g].arrival_s and its stage k
+ * (1..stages) becomes ready interval_s*(k-1) later.  Stage 1 = usap_create
+ * + usap_partition; stage k > 1 = usap_set_graph + usap_resume (nodal
+ * streams from k << 40), so the engine stays resident between stages.
+ * Graphs are admitted in arrival order while their footprints fit
+ * hbm_budget; up to `workers` ready stages of different graphs run at once,
+ * oldest first.  Each graph's final labels go to labels + label_off[g];
+ * stages[g*stages + k-1] records stage k.  Every result equals the one the
+ * graph gets served alone.  USAP_ESTATE if a graph failed (its stage record
+ * holds the error). */
+int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts* opt,
+               int32_t* labels, const int64_t* label_off, usap_serve_stage* stages,
+               usap_serve_stats* out);
 
 /* Picks the number of graphs in flight: partitions a sample of `sample`
  * graphs spread over the size range (0 = 24) with 2, 4, 8, ... workers (at

@@ -134,6 +134,67 @@ class UsapBatchStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class UsapServeGraph(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32),
+        ("E", C.c_int64),
+        ("src", C.c_void_p),
+        ("dst", C.c_void_p),
+        ("w", C.c_void_p),
+        ("arrival_s", C.c_double),
+        ("params", UsapParams),
+    ]
+
+
+class UsapServeOpts(C.Structure):
+    _fields_ = [
+        ("stages", C.c_int32),
+        ("interval_s", C.c_double),
+        ("workers", C.c_int32),
+        ("hbm_budget", C.c_int64),
+    ]
+
+
+class UsapServeStage(C.Structure):
+    _fields_ = [
+        ("graph", C.c_int32),
+        ("stage", C.c_int32),
+        ("t_ready", C.c_double),
+        ("t_start", C.c_double),
+        ("t_done", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_release", C.c_double),
+        ("S", C.c_double),
+        ("B", C.c_int32),
+        ("error", C.c_int32),
+        ("E", C.c_int64),
+        ("sweeps", C.c_int64),
+        ("nodal_proposals", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class UsapServeStats(C.Structure):
+    _fields_ = [
+        ("wall_s", C.c_double),
+        ("graphs_per_s", C.c_double),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("p50_first_ms", C.c_double),
+        ("p99_first_ms", C.c_double),
+        ("p50_resume_ms", C.c_double),
+        ("p99_resume_ms", C.c_double),
+        ("max_resident", C.c_int32),
+        ("failed", C.c_int32),
+        ("max_bytes", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class UsapKernelProf(C.Structure):
     _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
 
@@ -147,6 +208,8 @@ SIGNATURES = [
     ("usap_create", C.c_int, [C.POINTER(UsapGraph), C.POINTER(UsapParams), C.POINTER(_vp)]),
     ("usap_destroy", C.c_int, [_vp]),
     ("usap_last_error", C.c_char_p, [_vp]),
+    ("usap_set_graph", C.c_int, [_vp, C.POINTER(UsapGraph)]),
+    ("usap_resume", C.c_int, [_vp, C.c_uint64, _vp, C.POINTER(UsapStats)]),
     ("usap_set_labels", C.c_int, [_vp, _i32p, C.c_int32]),
     ("usap_get_labels", C.c_int, [_vp, _i32p]),
     ("usap_num_blocks", C.c_int, [_vp, C.POINTER(C.c_int32)]),
@@ -173,6 +236,9 @@ SIGNATURES = [
                                        C.POINTER(UsapBatchStats)]),
     ("usap_batch_autotune", C.c_int, [C.c_int32, _vp, _vp, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_int32), _vp, _vp]),
+    ("usap_footprint", C.c_int64, [C.c_int32, C.c_int64]),
+    ("usap_serve", C.c_int, [C.c_int32, _vp, C.POINTER(UsapServeOpts), _vp, _vp, _vp,
+                             C.POINTER(UsapServeStats)]),
     ("usap_golden_update", C.c_int, [_i64p, _f64p, _i64p, C.c_int64, C.c_double, C.c_int64,
                                      C.c_double, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -49,7 +49,7 @@ def build(verbose=False):
     o = os.path.join(OBJ_DIR, "engine.o")
     _run([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "engine.hip"), "-o", o])
     objs.append(o)
-    for src in ("host.cpp", "driver.cpp", "batch.cpp"):
+    for src in ("host.cpp", "driver.cpp", "batch.cpp", "serve.cpp"):
         o = os.path.join(OBJ_DIR, src.replace(".cpp", ".o"))
         _run(["g++", *CXX_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

@@ -70,6 +70,49 @@ double ms_since(clk::time_point t0) {
   return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
 }
 
+// The nodal phase at a fixed B (graph_parallel.hpp:381-478): sweeps of
+// batches until the moving-average convergence test fires (threshold thr) or
+// max_nodal_itr sweeps; the entropy is recomputed before each sweep
+// (prepare_batch) and after the last.  Advances nodal_stream by the batches
+// run; S receives the final MDL.
+int nodal_phase(usap_engine* e, const usap_params& p, int32_t batch, int32_t B, double thr,
+                uint64_t& nodal_stream, std::vector<double>& itr_dS, usap_stats& st, double& S) {
+  const int32_t W = p.avg_window;
+  const int64_t moves0 = st.accepted_moves;
+  int rc = USAP_OK;
+  std::fill(itr_dS.begin(), itr_dS.end(), 0.0);
+  for (int32_t itr = 0; itr < p.max_nodal_itr; ++itr) {
+    // prepare_batch
+    auto t0 = clk::now();
+    if ((rc = usap_entropy(e, &S, nullptr, nullptr))) return rc;
+    st.ms_entropy += ms_since(t0);
+    double mean = 0;
+    if (itr >= W - 1) {
+      for (int32_t i = itr - W + 1; i < itr; ++i) mean += itr_dS[i];
+      mean /= (double)(W - 1);
+    }
+    st.sweeps++;
+    t0 = clk::now();
+    // fetch_batch / nodal_move / update / check_batch (graph_parallel.hpp:397-468),
+    // the update step and its convergence test run on the device
+    usap_sweep sw{};
+    if ((rc = usap_nodal_sweep(e, batch, nodal_stream, S, mean, thr, itr >= W - 1, &sw)))
+      return rc;
+    nodal_stream += (uint64_t)sw.batches;
+    st.batches += sw.batches;
+    st.nodal_proposals += sw.proposals;
+    st.accepted_moves += sw.moves;
+    itr_dS[itr] = sw.dS_sum;
+    S = sw.S;
+    st.ms_nodal += ms_since(t0);
+    if (p.verbose > 1)
+      std::fprintf(stderr, "usap:   sweep %d B=%d S=%.3f moves=%lld %.1f ms\n", itr, B, S,
+                   (long long)(st.accepted_moves - moves0), ms_since(t0));
+    if (sw.converged) break;
+  }
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -121,7 +164,6 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
 
   int32_t batch = p.nodal_batch > 0 ? p.nodal_batch : N / p.num_batch_nodal_update;
   if (batch < 1) batch = 1;
-  const int32_t W = p.avg_window;
   std::vector<double> itr_dS((size_t)p.max_nodal_itr + 1, 0.0);
   Snap large, med, small;
   // snapshot slot of each partition id held by the golden-section bracket
@@ -151,39 +193,8 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     ph.B = B;
     ph.ms_merge = ms_since(t_phase);
     const auto t_nodal = clk::now();
-    std::fill(itr_dS.begin(), itr_dS.end(), 0.0);
-    for (int32_t itr = 0; itr < p.max_nodal_itr; ++itr) {
-      // prepare_batch
-      t0 = clk::now();
-      if ((rc = usap_entropy(e, &S, nullptr, nullptr))) break;
-      st.ms_entropy += ms_since(t0);
-      double mean = 0;
-      if (itr >= W - 1) {
-        for (int32_t i = itr - W + 1; i < itr; ++i) mean += itr_dS[i];
-        mean /= (double)(W - 1);
-      }
-      const bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
-      st.sweeps++;
-      t0 = clk::now();
-      // fetch_batch / nodal_move / update / check_batch (graph_parallel.hpp:397-468),
-      // the update step and its convergence test run on the device
-      usap_sweep sw{};
-      if ((rc = usap_nodal_sweep(e, batch, nodal_stream, S, mean, isf ? p.thr2 : p.thr1,
-                                 itr >= W - 1, &sw)))
-        break;
-      nodal_stream += (uint64_t)sw.batches;
-      st.batches += sw.batches;
-      st.nodal_proposals += sw.proposals;
-      st.accepted_moves += sw.moves;
-      itr_dS[itr] = sw.dS_sum;
-      S = sw.S;
-      const bool converged = sw.converged != 0;
-      st.ms_nodal += ms_since(t0);
-      if (p.verbose > 1)
-        std::fprintf(stderr, "usap:   sweep %d B=%d S=%.3f moves=%lld %.1f ms\n", itr, B, S,
-                     (long long)(st.accepted_moves - moves0), ms_since(t0));
-      if (rc || converged) break;
-    }
+    const bool isf = std::isfinite(large.S) && std::isfinite(med.S) && std::isfinite(small.S);
+    rc = nodal_phase(e, p, batch, B, isf ? p.thr2 : p.thr1, nodal_stream, itr_dS, st, S);
     if (rc) break;
     ph.ms_nodal = ms_since(t_nodal);
     // prepare_for_next
@@ -247,6 +258,40 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
       S = next.S;
       break;
     }
+  }
+  st.ms_total = ms_since(t_all);
+  st.S = S;
+  st.B = B;
+  st.rebuilds = usap::engine_rebuilds(e) - rebuilds0;
+  st.grows = usap::engine_grows(e) - grows0;
+  st.error = rc;
+  if (stats) *stats = st;
+  if (rc) return rc;
+  if (labels_out) rc = usap_get_labels(e, labels_out);
+  return rc;
+}
+
+int usap_resume(usap_engine* e, uint64_t stream0, int32_t* labels_out, usap_stats* stats) {
+  if (!e) return USAP_EINVAL;
+  const usap_graph* g = usap::engine_host_graph(e);
+  const usap_params& p = *usap::engine_params(e);
+  usap_stats st{};
+  int32_t B = 0;
+  int rc = usap_num_blocks(e, &B);
+  if (rc) return rc;
+  const int rebuilds0 = usap::engine_rebuilds(e);
+  const int64_t grows0 = usap::engine_grows(e);
+  const auto t_all = clk::now();
+  int32_t batch = p.nodal_batch > 0 ? p.nodal_batch : g->N / p.num_batch_nodal_update;
+  if (batch < 1) batch = 1;
+  std::vector<double> itr_dS((size_t)p.max_nodal_itr + 1, 0.0);
+  uint64_t nodal_stream = stream0;
+  double S = 0;
+  rc = nodal_phase(e, p, batch, B, p.thr2, nodal_stream, itr_dS, st, S);
+  if (rc == USAP_OK) {
+    const auto t0 = clk::now();
+    rc = usap_entropy(e, &S, nullptr, nullptr);
+    st.ms_entropy += ms_since(t0);
   }
   st.ms_total = ms_since(t_all);
   st.S = S;

@@ -1718,7 +1718,8 @@ struct DevSpM {
   int32_t* gmark = nullptr;  // [N]
   int32_t* gq = nullptr;     // [N]
   int32_t* gpend = nullptr;  // [N]
-  int32_t* ovf = nullptr;    // [3][kOvfCap]
+  int32_t* ovf = nullptr;    // [3][ovf_cap]
+  int32_t ovf_cap = 0;       // parked increments that fit
   GrowState* gst = nullptr;
   int64_t P = 0;      // allocated slots
   int64_t total = 0;  // slots in use after the last rebuild
@@ -1881,7 +1882,7 @@ MView view(const DevSpM& m, int32_t B) {
   v.gq = m.gq;
   v.gpend = m.gpend;
   v.ovf = m.ovf;
-  v.ovf_cap = kOvfCap;
+  v.ovf_cap = m.ovf_cap;
   v.gst = m.gst;
   v.B = B;
   v.bshift = bucket_shift(B);
@@ -2145,9 +2146,14 @@ int usap_destroy(usap_engine* e) {
   return USAP_OK;
 }
 
-int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
-  if (!g || !out) return USAP_EINVAL;
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Checks a graph (CSR/CSC consistent, ids in range, weights >= 0, total
+// weight < 2^31 since M counts and degrees are int32); *wtot = total weight.
+int check_graph(const usap_graph* g, int64_t* wtot) {
+  if (!g) return USAP_EINVAL;
   if (g->N < 1 || g->E < 0 || !g->out_ptr || !g->in_ptr || (g->E > 0 && (!g->out_idx || !g->in_idx)))
     return USAP_EINVAL;
   if ((g->out_w == nullptr) != (g->in_w == nullptr)) return USAP_EINVAL;
@@ -2155,6 +2161,166 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
   const int64_t E = g->E;
   if (g->out_ptr[0] != 0 || g->out_ptr[N] != E || g->in_ptr[0] != 0 || g->in_ptr[N] != E)
     return USAP_EINVAL;
+  int64_t w = E;
+  if (g->out_w) {
+    w = 0;
+    for (int64_t i = 0; i < E; ++i) {
+      if (g->out_w[i] < 0) return USAP_EINVAL;
+      w += g->out_w[i];
+    }
+  }
+  for (int64_t i = 0; i < E; ++i)
+    if (g->out_idx[i] < 0 || g->out_idx[i] >= N || g->in_idx[i] < 0 || g->in_idx[i] >= N)
+      return USAP_EINVAL;
+  if (w >= (int64_t)1 << 31) return USAP_EINVAL;
+  *wtot = w;
+  return USAP_OK;
+}
+
+#define TRY(x)          \
+  do {                  \
+    int _rc = (x);      \
+    if (_rc) return _rc; \
+  } while (0)
+
+// The graph's edges on the device: host copy, CSR/CSC, edge sources, the
+// lambda table over [0, wtot] and the global tables of large nodes.  Frees
+// the previous graph's arrays first (same N).
+int load_graph(usap_engine* e, const usap_graph* g, int64_t wtot) {
+  const int32_t N = g->N;
+  const int64_t E = g->E;
+  e->N = N;
+  e->E = E;
+  e->weighted = g->out_w != nullptr;
+  e->h_out_ptr.assign(g->out_ptr, g->out_ptr + N + 1);
+  e->h_in_ptr.assign(g->in_ptr, g->in_ptr + N + 1);
+  e->h_out_idx.assign(g->out_idx, g->out_idx + E);
+  e->h_in_idx.assign(g->in_idx, g->in_idx + E);
+  e->h_out_w.clear();
+  e->h_in_w.clear();
+  if (e->weighted) {
+    e->h_out_w.assign(g->out_w, g->out_w + E);
+    e->h_in_w.assign(g->in_w, g->in_w + E);
+  }
+  e->Wtot = wtot;
+  e->hg.N = N;
+  e->hg.E = E;
+  e->hg.out_ptr = e->h_out_ptr.data();
+  e->hg.in_ptr = e->h_in_ptr.data();
+  e->hg.out_idx = e->h_out_idx.data();
+  e->hg.in_idx = e->h_in_idx.data();
+  e->hg.out_w = e->weighted ? e->h_out_w.data() : nullptr;
+  e->hg.in_w = e->weighted ? e->h_in_w.data() : nullptr;
+  for (int32_t** q : {&e->out_idx, &e->in_idx, &e->out_src, &e->out_w, &e->in_w, &e->big_tab})
+    dfree(e, *q);
+  for (int64_t** q : {&e->out_ptr, &e->in_ptr, &e->lam, &e->big_off}) dfree(e, *q);
+  TRY(dalloc(e, &e->out_ptr, N + 1));
+  TRY(dalloc(e, &e->in_ptr, N + 1));
+  TRY(dalloc(e, &e->out_idx, E));
+  TRY(dalloc(e, &e->in_idx, E));
+  TRY(dalloc(e, &e->out_src, E));
+  if (e->weighted) {
+    TRY(dalloc(e, &e->out_w, E));
+    TRY(dalloc(e, &e->in_w, E));
+  }
+  TRY(dalloc(e, &e->lam, wtot + 1));
+  std::vector<int64_t> lam((size_t)wtot + 1);
+  usap_lambda_table(wtot, lam.data());
+  std::vector<int32_t> src((size_t)E);
+  for (int32_t x = 0; x < N; ++x)
+    for (int64_t j = g->out_ptr[x]; j < g->out_ptr[x + 1]; ++j) src[j] = x;
+  // global-memory neighbour tables of the nodes too large for LDS
+  std::vector<int64_t> big((size_t)N, -1);
+  int64_t big_total = 0;
+  for (int32_t x = 0; x < N; ++x) {
+    const int64_t k = (g->out_ptr[x + 1] - g->out_ptr[x]) + (g->in_ptr[x + 1] - g->in_ptr[x]);
+    if (k > kTabMaxDeg) {
+      int64_t cap = 1;
+      while (cap < 2 * k) cap <<= 1;
+      big[x] = big_total;
+      big_total += 3 * cap;
+    }
+  }
+  TRY(dalloc(e, &e->big_off, N));
+  TRY(dalloc(e, &e->big_tab, big_total));
+  hipStream_t st = e->stream;
+  HIPCHK(hipMemcpyAsync(e->out_ptr, g->out_ptr, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(e->in_ptr, g->in_ptr, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
+  if (E > 0) {
+    HIPCHK(hipMemcpyAsync(e->out_idx, g->out_idx, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->in_idx, g->in_idx, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(e->out_src, src.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+    if (e->weighted) {
+      HIPCHK(hipMemcpyAsync(e->out_w, g->out_w, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(e->in_w, g->in_w, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
+    }
+  }
+  HIPCHK(hipMemcpyAsync(e->lam, lam.data(), sizeof(int64_t) * (wtot + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, st));
+  // the host vectors above are released when this returns: wait for the copies
+  HIPCHK(hipStreamSynchronize(st));
+  return USAP_OK;
+}
+
+// Device arrays sized by N only (labels, degrees, row tables, batch buffers).
+int alloc_state(usap_engine* e) {
+  const int32_t N = e->N;
+  TRY(dalloc(e, &e->labels, N));
+  TRY(dalloc(e, &e->d_out, N));
+  TRY(dalloc(e, &e->d_in, N));
+  TRY(dalloc(e, &e->ecnt_out, N));
+  TRY(dalloc(e, &e->ecnt_in, N));
+  TRY(dalloc(e, &e->bsize, N));
+  for (DevSpM* m : {&e->M, &e->Mt}) {
+    TRY(dalloc(e, &m->off, (size_t)N + 1));
+    TRY(dalloc(e, &m->cap, N));
+    TRY(dalloc(e, &m->nkeys, N));
+    TRY(dalloc(e, &m->gmark, N));
+    TRY(dalloc(e, &m->gq, N));
+    TRY(dalloc(e, &m->gpend, N));
+    m->ovf_cap = (int32_t)std::min<int64_t>(kOvfCap, 2 * e->E + 4096);
+    TRY(dalloc(e, &m->ovf, (size_t)3 * m->ovf_cap));
+    TRY(dalloc(e, &m->gst, 1));
+  }
+  TRY(dalloc(e, &e->tmpM, (size_t)N + 1));
+  TRY(dalloc(e, &e->tmpT, (size_t)N + 1));
+  TRY(dalloc(e, &e->best_s, N));
+  TRY(dalloc(e, &e->best_hi, N));
+  TRY(dalloc(e, &e->best_lo, N));
+  TRY(dalloc(e, &e->map, N));
+  TRY(dalloc(e, &e->prop_s, N));
+  TRY(dalloc(e, &e->prop_hi, N));
+  TRY(dalloc(e, &e->prop_lo, N));
+  TRY(dalloc(e, &e->prop_inf, N));
+  TRY(dalloc(e, &e->prop_acc, N));
+  TRY(dalloc(e, &e->status, 1));
+  TRY(dalloc(e, &e->part, blocks_for(N, kNodalWaves)));
+  TRY(dalloc(e, &e->sweep, 1));
+  TRY(dalloc(e, &e->prop_r, N));
+  TRY(dalloc(e, &e->build_flags, 1));
+  TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
+  e->h_sweep = (Sweep*)pin_get(sizeof(Sweep));
+  e->h_ent_part = (long*)pin_get(sizeof(long) * 2 * kEntropyBlocks);
+  if (!e->h_sweep || !e->h_ent_part) return set_err(e, USAP_ENOMEM, "hipHostMalloc");
+  hipStream_t st = e->stream;
+  HIPCHK(hipMemsetAsync(e->status, 0, sizeof(Status), st));
+  HIPCHK(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
+  std::memset(e->h_sweep, 0, sizeof(Sweep));
+  for (hipEvent_t& ev : e->ev) HIPCHK(hipEventCreate(&ev));
+  return USAP_OK;
+}
+
+#undef TRY
+
+}  // namespace
+
+extern "C" {
+
+int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
+  if (!g || !out) return USAP_EINVAL;
+  *out = nullptr;
+  int64_t wtot = 0;
+  if (int rc = check_graph(g, &wtot)) return rc;
   usap_engine* e = new (std::nothrow) usap_engine();
   if (!e) return USAP_ENOMEM;
   if (p)
@@ -2166,49 +2332,11 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     delete e;
     return USAP_EINVAL;
   }
-  e->N = N;
-  e->E = E;
-  e->weighted = g->out_w != nullptr;
-  e->h_out_ptr.assign(g->out_ptr, g->out_ptr + N + 1);
-  e->h_in_ptr.assign(g->in_ptr, g->in_ptr + N + 1);
-  e->h_out_idx.assign(g->out_idx, g->out_idx + E);
-  e->h_in_idx.assign(g->in_idx, g->in_idx + E);
-  int64_t wtot = E;
-  if (e->weighted) {
-    e->h_out_w.assign(g->out_w, g->out_w + E);
-    e->h_in_w.assign(g->in_w, g->in_w + E);
-    wtot = 0;
-    for (int64_t i = 0; i < E; ++i) {
-      if (g->out_w[i] < 0) {
-        delete e;
-        return USAP_EINVAL;
-      }
-      wtot += g->out_w[i];
-    }
-  }
-  for (int64_t i = 0; i < E; ++i)
-    if (g->out_idx[i] < 0 || g->out_idx[i] >= N || g->in_idx[i] < 0 || g->in_idx[i] >= N) {
-      delete e;
-      return USAP_EINVAL;
-    }
-  if (wtot >= (int64_t)1 << 31) {  // M counts and degrees are int32
-    delete e;
-    return USAP_EINVAL;
-  }
-  e->Wtot = wtot;
-  e->hg.N = N;
-  e->hg.E = E;
-  e->hg.out_ptr = e->h_out_ptr.data();
-  e->hg.in_ptr = e->h_in_ptr.data();
-  e->hg.out_idx = e->h_out_idx.data();
-  e->hg.in_idx = e->h_in_idx.data();
-  e->hg.out_w = e->weighted ? e->h_out_w.data() : nullptr;
-  e->hg.in_w = e->weighted ? e->h_in_w.data() : nullptr;
-
-  int rc = USAP_OK;
+  e->N = g->N;
+  e->E = g->E;
   hipError_t hst = hipSetDevice(e->p.device);
   if (hst != hipSuccess) {
-    rc = set_err(e, USAP_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(hst));
+    const int rc = set_err(e, USAP_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(hst));
     usap_destroy(e);
     return rc;
   }
@@ -2219,116 +2347,33 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     usap_destroy(e);
     return USAP_EHIP;
   }
-#define TRY(x)            \
-  do {                    \
-    rc = (x);             \
-    if (rc) goto fail;    \
-  } while (0)
-#define TRYH(x)                                                                        \
-  do {                                                                                 \
-    hipError_t _h = (x);                                                               \
-    if (_h != hipSuccess) {                                                            \
-      rc = set_err(e, USAP_EHIP, std::string(#x) + ": " + hipGetErrorString(_h));      \
-      goto fail;                                                                       \
-    }                                                                                  \
-  } while (0)
-  {
-    TRY(dalloc(e, &e->out_ptr, N + 1));
-    TRY(dalloc(e, &e->in_ptr, N + 1));
-    TRY(dalloc(e, &e->out_idx, E));
-    TRY(dalloc(e, &e->in_idx, E));
-    TRY(dalloc(e, &e->out_src, E));
-    if (e->weighted) {
-      TRY(dalloc(e, &e->out_w, E));
-      TRY(dalloc(e, &e->in_w, E));
-    }
-    TRY(dalloc(e, &e->lam, wtot + 1));
-    TRY(dalloc(e, &e->labels, N));
-    TRY(dalloc(e, &e->d_out, N));
-    TRY(dalloc(e, &e->d_in, N));
-    TRY(dalloc(e, &e->ecnt_out, N));
-    TRY(dalloc(e, &e->ecnt_in, N));
-    TRY(dalloc(e, &e->bsize, N));
-    for (DevSpM* m : {&e->M, &e->Mt}) {
-      TRY(dalloc(e, &m->off, (size_t)N + 1));
-      TRY(dalloc(e, &m->cap, N));
-      TRY(dalloc(e, &m->nkeys, N));
-      TRY(dalloc(e, &m->gmark, N));
-      TRY(dalloc(e, &m->gq, N));
-      TRY(dalloc(e, &m->gpend, N));
-      TRY(dalloc(e, &m->ovf, (size_t)3 * kOvfCap));
-      TRY(dalloc(e, &m->gst, 1));
-    }
-    TRY(dalloc(e, &e->tmpM, (size_t)N + 1));
-    TRY(dalloc(e, &e->tmpT, (size_t)N + 1));
-    TRY(dalloc(e, &e->best_s, N));
-    TRY(dalloc(e, &e->best_hi, N));
-    TRY(dalloc(e, &e->best_lo, N));
-    TRY(dalloc(e, &e->map, N));
-    TRY(dalloc(e, &e->prop_s, N));
-    TRY(dalloc(e, &e->prop_hi, N));
-    TRY(dalloc(e, &e->prop_lo, N));
-    TRY(dalloc(e, &e->prop_inf, N));
-    TRY(dalloc(e, &e->prop_acc, N));
-    TRY(dalloc(e, &e->status, 1));
-    TRY(dalloc(e, &e->part, blocks_for(N, kNodalWaves)));
-    TRY(dalloc(e, &e->sweep, 1));
-    TRY(dalloc(e, &e->prop_r, N));
-    TRY(dalloc(e, &e->build_flags, 1));
-    TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
-    e->h_sweep = (Sweep*)pin_get(sizeof(Sweep));
-    e->h_ent_part = (long*)pin_get(sizeof(long) * 2 * kEntropyBlocks);
-    if (!e->h_sweep || !e->h_ent_part) {
-      rc = set_err(e, USAP_ENOMEM, "hipHostMalloc");
-      goto fail;
-    }
-    // lambda table
-    std::vector<int64_t> lam((size_t)wtot + 1);
-    usap_lambda_table(wtot, lam.data());
-    std::vector<int32_t> src((size_t)E);
-    for (int32_t x = 0; x < N; ++x)
-      for (int64_t j = g->out_ptr[x]; j < g->out_ptr[x + 1]; ++j) src[j] = x;
-    // global-memory neighbour tables of the nodes too large for LDS
-    std::vector<int64_t> big((size_t)N, -1);
-    int64_t big_total = 0;
-    for (int32_t x = 0; x < N; ++x) {
-      const int64_t k = (g->out_ptr[x + 1] - g->out_ptr[x]) + (g->in_ptr[x + 1] - g->in_ptr[x]);
-      if (k > kTabMaxDeg) {
-        int64_t cap = 1;
-        while (cap < 2 * k) cap <<= 1;
-        big[x] = big_total;
-        big_total += 3 * cap;
-      }
-    }
-    TRY(dalloc(e, &e->big_off, N));
-    TRY(dalloc(e, &e->big_tab, big_total));
-    hipStream_t st = e->stream;
-    TRYH(hipMemcpyAsync(e->out_ptr, g->out_ptr, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
-    TRYH(hipMemcpyAsync(e->in_ptr, g->in_ptr, sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
-    if (E > 0) {
-      TRYH(hipMemcpyAsync(e->out_idx, g->out_idx, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
-      TRYH(hipMemcpyAsync(e->in_idx, g->in_idx, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
-      TRYH(hipMemcpyAsync(e->out_src, src.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
-      if (e->weighted) {
-        TRYH(hipMemcpyAsync(e->out_w, g->out_w, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
-        TRYH(hipMemcpyAsync(e->in_w, g->in_w, sizeof(int32_t) * E, hipMemcpyHostToDevice, st));
-      }
-    }
-    TRYH(hipMemcpyAsync(e->lam, lam.data(), sizeof(int64_t) * (wtot + 1), hipMemcpyHostToDevice, st));
-    TRYH(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, st));
-    TRYH(hipMemsetAsync(e->status, 0, sizeof(Status), st));
-    TRYH(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
-    std::memset(e->h_sweep, 0, sizeof(Sweep));
-    for (hipEvent_t& ev : e->ev) TRYH(hipEventCreate(&ev));
-    TRYH(hipStreamSynchronize(st));
+  int rc = alloc_state(e);
+  if (rc == USAP_OK) rc = load_graph(e, g, wtot);
+  if (rc != USAP_OK) {
+    usap_destroy(e);
+    return rc;
   }
-#undef TRY
-#undef TRYH
   *out = e;
   return USAP_OK;
-fail:
-  usap_destroy(e);
-  return rc;
+}
+
+int usap_set_graph(usap_engine* e, const usap_graph* g) {
+  if (!e) return USAP_EINVAL;
+  int64_t wtot = 0;
+  if (int rc = check_graph(g, &wtot)) return rc;
+  if (g->N != e->N) return set_err(e, USAP_EINVAL, "usap_set_graph: node count differs");
+  HIPCHK(hipSetDevice(e->p.device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const int64_t cap = std::min<int64_t>(kOvfCap, 2 * g->E + 4096);
+  for (DevSpM* m : {&e->M, &e->Mt})
+    if (cap > m->ovf_cap) {  // overflow buffers follow the edge count
+      dfree(e, m->ovf);
+      m->ovf_cap = (int32_t)cap;
+      if (int rc = dalloc(e, &m->ovf, (size_t)3 * m->ovf_cap)) return rc;
+    }
+  if (int rc = load_graph(e, g, wtot)) return rc;
+  e->merge_B = -1;
+  return e->has_labels ? rebuild_M(e) : USAP_OK;
 }
 
 int usap_set_labels(usap_engine* e, const int32_t* labels, int32_t B) {
@@ -2629,7 +2674,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
     gs.gq = ms[i]->gq;
     gs.gpend = ms[i]->gpend;
     gs.ovf = ms[i]->ovf;
-    gs.ovf_cap = kOvfCap;
+    gs.ovf_cap = ms[i]->ovf_cap;
     gs.gst = ms[i]->gst;
     gs.P = ms[i]->P;
   }

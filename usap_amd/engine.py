@@ -152,6 +152,19 @@ class Engine:
         self._ck(lib().usap_partition(self.h, lab.ctypes.data, C.byref(st)))
         return lab, st.as_dict()
 
+    def set_graph(self, graph):
+        """Replace the edges (same N), keeping labels and B (usap_set_graph)."""
+        gs = graph.as_struct()
+        self._ck(lib().usap_set_graph(self.h, C.byref(gs)))
+        self.graph, self._gs = graph, gs
+
+    def resume(self, stream0):
+        """Nodal phase at the current partition (usap_resume): (labels, stats)."""
+        lab = np.zeros(self.graph.N, np.int32)
+        st = UsapStats()
+        self._ck(lib().usap_resume(self.h, stream0, lab.ctypes.data, C.byref(st)))
+        return lab, st.as_dict()
+
     def partition_log(self):
         n = C.c_int32()
         self._ck(lib().usap_partition_log(self.h, 0, None, C.byref(n)))
