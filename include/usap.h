@@ -74,6 +74,9 @@ typedef struct usap_params {
   int32_t device;                  /* HIP device ordinal */
   int32_t verbose;                 /* print per-phase progress to stderr */
   int32_t slab_slack;              /* hashed M row capacity = pow2 >= slack*edges+16 (4; 1..16) */
+  int32_t graphs;                  /* 1: replay each nodal sweep as a captured HIP graph (0 =
+                                      default; only with one engine per process: another
+                                      thread's HIP calls can invalidate a capture) */
 } usap_params;
 
 /* Partition statistics (usap_partition).  Times are host wall-clock ms. */
@@ -216,6 +219,7 @@ typedef struct usap_phase {
   double k_apply_ms;
   double k_merge_ms;
   double k_build_ms;
+  double k_grow_ms;
   int32_t rebuilds;    /* full M rebuilds in this phase */
 } usap_phase;
 
@@ -319,7 +323,9 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
                         int32_t max_workers, int32_t sample, int32_t* workers, int32_t* tried,
                         double* rates);
 
-/* Kernel profile: HIP-event time on the engine's stream, launches, and the
+/* Kernel profile: time on the engine's stream (nodal, apply and grow: each
+ * kernel's own execution, from events attached to its launch; the others:
+ * events around their launches), launches, and the
  * algorithmic bytes the kernel moved (reads + writes the algorithm needs,
  * each counted once; 0 where not tallied). */
 #define USAP_K_NODAL 0   /* nodal proposals */
@@ -327,7 +333,8 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
 #define USAP_K_MERGE 2   /* merge proposals */
 #define USAP_K_ENTROPY 3 /* MDL reduction */
 #define USAP_K_BUILD 4   /* M rebuild (all its kernels) */
-#define USAP_NKERNELS 5
+#define USAP_K_GROW 5    /* row growth / overflow replay between nodal batches */
+#define USAP_NKERNELS 6
 typedef struct usap_kernel_prof {
   int64_t launches;
   double ms;

@@ -86,3 +86,22 @@ def test_partition_with_tight_slabs(usap, orc, gname):
     B, olab, ost = o.partition()
     assert st["B"] == B and st["S"] == ost["S"]
     np.testing.assert_array_equal(lab, olab)
+
+
+@pytest.mark.parametrize("gname,slack", [("c1", 4), ("weighted", 1)])
+def test_graph_replay_equals_direct_launches(usap, gname, slack):
+    """Sweeps replayed from a captured HIP graph (the default) and launched
+    kernel by kernel give the same partition; the captured sweep is replayed
+    many times with host work (entropy, merges, snapshots) in between, and
+    re-captured whenever B or the M pools change."""
+    g = usap.config_graph("c1") if gname == "c1" else random_graph(
+        700, 9000, seed=5, weighted=True, self_loops=True, hubs=2, isolated=7)
+    res = []
+    for graphs in (1, 0):
+        eng = usap.Engine(g, seed=6, init=1, ds_mode=1, slab_slack=slack, graphs=graphs)
+        lab, st = eng.partition()
+        eng.close()
+        res.append((lab, st))
+    np.testing.assert_array_equal(res[0][0], res[1][0])
+    for k in ("S", "B", "sweeps", "batches", "accepted_moves"):
+        assert res[0][1][k] == res[1][1][k]

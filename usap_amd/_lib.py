@@ -58,6 +58,7 @@ class UsapParams(C.Structure):
         ("device", C.c_int32),
         ("verbose", C.c_int32),
         ("slab_slack", C.c_int32),
+        ("graphs", C.c_int32),
     ]
 
 
@@ -99,6 +100,7 @@ class UsapPhase(C.Structure):
         ("k_apply_ms", C.c_double),
         ("k_merge_ms", C.c_double),
         ("k_build_ms", C.c_double),
+        ("k_grow_ms", C.c_double),
         ("rebuilds", C.c_int32),
     ]
 
@@ -199,7 +201,7 @@ class UsapKernelProf(C.Structure):
     _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
 
 
-KERNELS = ["nodal", "apply", "merge", "entropy", "build"]
+KERNELS = ["nodal", "apply", "merge", "entropy", "build", "grow"]
 
 
 # (name, restype, argtypes) of every exported entry point of include/usap.h

@@ -6,7 +6,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import UsapBatchStats, UsapGraph, UsapParams, UsapStats, check, lib
+from ._lib import UsapBatchStats, UsapError, UsapGraph, UsapParams, UsapStats, lib
 from .engine import default_params
 from .graph import generate_dcsbm
 
@@ -54,9 +54,10 @@ def partition_batch(graphs, params, workers=0):
                                     workers, C.byref(bs))
     out = bs.as_dict()
     st = [stats[i].as_dict() for i in range(n)]
-    if rc != 0 and out["failed"] == 0:
-        check(rc)
     out["rc"] = rc
+    out["error"] = lib().usap_last_error(None).decode() if rc != 0 else ""
+    if rc != 0 and out["failed"] == 0:
+        raise UsapError(rc, out["error"])
     return [labels[off[i]:off[i + 1]].copy() for i in range(n)], st, out
 
 
@@ -67,7 +68,9 @@ def autotune(graphs, params, max_workers=0, sample=0):
     w = C.c_int32()
     tried = np.zeros(16, np.int32)
     rates = np.zeros(16, np.float64)
-    check(lib().usap_batch_autotune(n, gs, ps, max_workers, sample, C.byref(w),
-                                    tried.ctypes.data, rates.ctypes.data))
+    rc = lib().usap_batch_autotune(n, gs, ps, max_workers, sample, C.byref(w),
+                                   tried.ctypes.data, rates.ctypes.data)
+    if rc != 0:
+        raise UsapError(rc, lib().usap_last_error(None).decode())
     trials = [(int(a), float(b)) for a, b in zip(tried, rates) if a > 0]
     return w.value, trials

@@ -14,9 +14,11 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <thread>
 #include <vector>
 
+#include "engine_internal.hpp"
 #include "usap.h"
 
 namespace {
@@ -46,6 +48,9 @@ int32_t run_pool(const std::vector<int32_t>& idx, const usap_graph* graphs,
       std::memset(&st, 0, sizeof(st));
       int rc = usap_create(&graphs[i], &params[i], &e);
       if (rc == USAP_OK) rc = usap_partition(e, labels + label_off[i], &st);
+      if (rc != USAP_OK)
+        usap::set_detached_error("usap_partition_batch: graph " + std::to_string(i) + ": error " +
+                                 std::to_string(rc) + ": " + (e ? usap_last_error(e) : "usap_create failed"));
       if (e) usap_destroy(e);
       st.error = rc;
       if (stats) stats[i] = st;
@@ -94,6 +99,7 @@ int usap_partition_batch(int32_t n, const usap_graph* graphs, const usap_params*
                          int32_t workers, usap_batch_stats* out) {
   if (!valid(n, graphs, params) || (n > 0 && (!labels || !label_off))) return USAP_EINVAL;
   if (workers < 0) return USAP_EINVAL;
+  usap::clear_detached_error();
   if (workers == 0) {
     const int rc = usap_batch_autotune(n, graphs, params, 0, 0, &workers, nullptr, nullptr);
     if (rc != USAP_OK) return rc;
@@ -121,6 +127,7 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
     return USAP_EINVAL;
   if (max_workers == 0) max_workers = 16;
   if (sample == 0) sample = 24;
+  usap::clear_detached_error();
   *workers = 1;
   if (n == 0) return USAP_OK;
   // the sample spans the size range: every k-th graph of the size order

@@ -211,6 +211,7 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     ph.k_apply_ms = kp1[USAP_K_APPLY].ms - kp0[USAP_K_APPLY].ms;
     ph.k_merge_ms = kp1[USAP_K_MERGE].ms - kp0[USAP_K_MERGE].ms;
     ph.k_build_ms = kp1[USAP_K_BUILD].ms - kp0[USAP_K_BUILD].ms;
+    ph.k_grow_ms = kp1[USAP_K_GROW].ms - kp0[USAP_K_GROW].ms;
     ph.rebuilds = usap::engine_rebuilds(e) - rb0;
     plog.push_back(ph);
     t0 = clk::now();
@@ -250,10 +251,10 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     if (p.verbose)
       std::fprintf(stderr,
                    "usap: phase %lld B=%d S=%.6f bracket=[%lld,%lld,%lld] next B=%lld sweeps=%d "
-                   "ms merge=%.1f nodal=%.1f k_nodal=%.1f k_apply=%.1f build=%.1f rebuilds=%d\n",
+                   "ms merge=%.1f nodal=%.1f k_nodal=%.1f k_apply=%.1f k_grow=%.1f build=%.1f rebuilds=%d\n",
                    (long long)phase, ph.B, ph.S, (long long)large.B, (long long)med.B,
                    (long long)small.B, (long long)(next.B - B_to_merge), ph.sweeps, ph.ms_merge,
-                   ph.ms_nodal, ph.k_nodal_ms, ph.k_apply_ms, ph.k_build_ms, ph.rebuilds);
+                   ph.ms_nodal, ph.k_nodal_ms, ph.k_apply_ms, ph.k_grow_ms, ph.k_build_ms, ph.rebuilds);
     if (gr == 1) {
       S = next.S;
       break;

@@ -10,8 +10,8 @@
 //                   keys int32 (-1 empty), counts int32.
 // Keys are never deleted inside a sweep (a count may drop to 0).  Each row
 // also keeps kNBK bucket sums (sum of its counts per key range) for the
-// two-level inverse-CDF pick.  A row whose key count passes 3/4 of its slab
-// is queued; an increment that finds no free slot is parked in an overflow
+// two-level inverse-CDF pick.  A row whose key count passes 1/4 of its slab
+// during nodal batches is queued; an increment that finds no free slot is parked in an overflow
 // buffer; between two batches k_grow compacts queued rows in place or moves
 // them to a larger slab at the pool's tail and replays the parked increments.
 // The host rebuilds M from the labels only when the pool runs out.
@@ -40,6 +40,7 @@
 // and degrees do not change cancel exactly in integer arithmetic, so a nodal
 // proposal costs O(distinct neighbour blocks) lookups and a merge proposal
 // O(min row length) lookups, instead of the reference's O(B) scans.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,6 +90,7 @@ struct Status {
 // graph_parallel.hpp:433-458, run by the last block of each k_apply).
 struct Sweep {
   unsigned long long cum_lo, cum_hi;  // exact cumulative dS of the sweep (int128)
+  unsigned long long stream0;         // random stream of the sweep's first batch
   double S;                           // running S: S += dS_sum after every batch
   double mean;                        // moving average of earlier sweeps' dS
   double thr;                         // delta_entropy_threshold1 or 2
@@ -237,7 +239,9 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
       k = atomicCAS(&m.key[o + h], -1, b);
       if (k == -1) {
         const int32_t n = atomicAdd(&m.nkeys[a], 1) + 1;
-        if (4 * n > 3 * c) {
+        // past 1/4 load during nodal batches (short probe chains for the
+        // lookups, most of which miss) or 3/4 in a build: grow / rebuild
+        if (kBuckets ? 4 * n > c : 4 * n > 3 * c) {
           if (!kBuckets) {
             atomicOr(flags, kFlagLoad);
           } else if (atomicCAS(&m.gmark[a], 0, 1) == 0) {  // queue the row once
@@ -298,6 +302,14 @@ __device__ __forceinline__ long wave_sum(long v) {
   return v;
 }
 
+// v of lane l (l the same in every lane)
+__device__ __forceinline__ long read_lane(long v, int l) {
+  const unsigned long long u = (unsigned long long)v;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return (long)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ i128 wave_sum128(i128 v) {
   unsigned long lo = (unsigned long)(u128)v;
   long hi = (long)(v >> 64);
@@ -322,30 +334,33 @@ __device__ __forceinline__ long wave_incl_scan(long v) {
   return v;
 }
 
+// The terms read the lambda table unconditionally (lambda of m <= 0 reads
+// lam[0] = 0) and select the result, so the table loads of a whole dS are
+// issued together: a load under `if (m > 0)` would sit in its own branch and
+// wait for every earlier term's loads first.
+__device__ __forceinline__ int64_t lam_of(const int64_t* lam, int64_t m) {
+  return lam[m > 0 ? m : 0];
+}
+
 // m * lambda(m): one L term
 __device__ __forceinline__ i128 Lterm(const int64_t* lam, int64_t m) {
-  return m > 0 ? (i128)m * (i128)lam[m] : (i128)0;
+  const int64_t L = lam_of(lam, m);
+  return m > 0 ? (i128)m * (i128)L : (i128)0;
 }
 
 // m * lambda(d): one degree-factor term; inf when m > 0 and d == 0
 __device__ __forceinline__ i128 Pterm(const int64_t* lam, int64_t m, int64_t d, bool& inf) {
-  if (m <= 0) return 0;
-  if (d <= 0) {
-    inf = true;
-    return 0;
-  }
-  return (i128)m * (i128)lam[d];
+  const int64_t L = lam_of(lam, d);
+  inf = inf || (m > 0 && d <= 0);
+  return (m > 0 && d > 0) ? (i128)m * (i128)L : (i128)0;
 }
 
 // T(m, x, y) = m*(lambda(m)-lambda(x)-lambda(y)); inf when m > 0 and x*y == 0
 __device__ __forceinline__ i128 Tterm(const int64_t* lam, int64_t m, int64_t x, int64_t y,
                                       bool& inf) {
-  if (m <= 0) return 0;
-  if (x <= 0 || y <= 0) {
-    inf = true;
-    return 0;
-  }
-  return (i128)m * (i128)(lam[m] - lam[x] - lam[y]);
+  const int64_t Lm = lam_of(lam, m), Lx = lam_of(lam, x), Ly = lam_of(lam, y);
+  inf = inf || (m > 0 && (x <= 0 || y <= 0));
+  return (m > 0 && x > 0 && y > 0) ? (i128)m * (i128)(Lm - Lx - Ly) : (i128)0;
 }
 
 // per-lane part of sum_{lo <= key <= x, key != excl} M[a][key]
@@ -366,8 +381,8 @@ __device__ __forceinline__ long count_range_lane(const MView& m, int64_t o, int3
 }
 
 // scr[k - lo] += M[a][k] for lo <= k < hi, from row R of a: a direct row is
-// read at columns [lo, hi), a hashed row's slab is scanned four slots per lane
-// at a time (all loads in flight before the LDS adds).
+// read at columns [lo, hi), a hashed row's slab is scanned eight slots per
+// lane at a time (all loads of a pass in flight before the LDS adds).
 __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, int32_t hi,
                                            int32_t* scr, long& bytes) {
   const int lane = lane_id();
@@ -376,10 +391,10 @@ __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, in
     bytes += 4 * (long)(hi - lo);
     return;
   }
-  for (int32_t base = 0; base < R.c; base += 4 * kWave) {
-    int32_t k[4], v[4];
+  for (int32_t base = 0; base < R.c; base += 8 * kWave) {
+    int32_t k[8], v[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 8; ++q) {
       const int32_t j = base + q * kWave + lane;
       k[q] = -1;
       v[q] = 0;
@@ -389,10 +404,44 @@ __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, in
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 8; ++q)
       if (k[q] >= lo && k[q] < hi && v[q] != 0) atomicAdd(&scr[k[q] - lo], v[q]);
   }
   bytes += 8 * (long)R.c;
+}
+
+// gather_row for row u of M and row u of M^T together: the passes over both
+// slabs are interleaved so their loads are in flight at the same time.
+__device__ __forceinline__ void gather_rows(const MView& M, Row RM, const MView& Mt, Row RT,
+                                            int32_t lo, int32_t hi, int32_t* scr, long& bytes) {
+  if (RM.c == M.B || RT.c == Mt.B) {  // a direct row: its range is one short pass
+    gather_row(M, RM, lo, hi, scr, bytes);
+    gather_row(Mt, RT, lo, hi, scr, bytes);
+    return;
+  }
+  const int lane = lane_id();
+  const int32_t cmax = max(RM.c, RT.c);
+  for (int32_t base = 0; base < cmax; base += 4 * kWave) {
+    int32_t k[8], v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const bool t = q >= 4;
+      const Row& R = t ? RT : RM;
+      const int32_t* key = t ? Mt.key : M.key;
+      const int32_t* val = t ? Mt.val : M.val;
+      const int32_t j = base + (q & 3) * kWave + lane;
+      k[q] = -1;
+      v[q] = 0;
+      if (j < R.c) {
+        k[q] = key[R.o + j];
+        v[q] = val[R.o + j];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (k[q] >= lo && k[q] < hi && v[q] != 0) atomicAdd(&scr[k[q] - lo], v[q]);
+  }
+  bytes += 8 * ((long)RM.c + RT.c);
 }
 
 // Index of the t-th unit in the LDS weights scr[0, width) (t < their sum):
@@ -428,9 +477,11 @@ __device__ __forceinline__ int32_t scan_scratch(const int32_t* scr, int32_t widt
 // running sum exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact
 // integer weights).  The caller guarantees t < total weight.  Two levels: the
 // rows' bucket sums (all loaded at once) locate the key bucket [lo, hi); the
-// bucket's weights then come from one lookup per key when it is at most a
-// wavefront wide, else they are gathered into the wavefront's LDS scratch
-// `scr` (scr_cap ints) and searched with one scan.
+// bucket's weights then come from one load per key when both rows are direct
+// and the bucket is at most a wavefront wide, else they are gathered into the
+// wavefront's LDS scratch `scr` (scr_cap ints) -- a hashed row by scanning its
+// slab, which costs a few loads per lane where looking up each absent key
+// would walk a probe chain -- and searched with one scan.
 __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
                                    int32_t excl, int32_t* scr, int32_t scr_cap, long& bytes) {
   const int lane = lane_id();
@@ -475,14 +526,11 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
   const int32_t lo = jb << sh;
   const int32_t hi = min(B, lo + (1 << sh));
   const int32_t width = hi - lo;
-  if (width <= kWave) {
+  if (width <= kWave && RM.c == B && RT.c == B) {  // direct rows: one load per key
     long w = 0;
     const int32_t i = lo + lane;
-    if (lane < width && i != excl) {
-      const Probe p = m_probe(M, RM, i), pt = m_probe(Mt, RT, i);
-      w = (long)m_value(M, RM, i, p) + (long)m_value(Mt, RT, i, pt);
-    }
-    bytes += 16 * (long)width;
+    if (lane < width && i != excl) w = (long)M.val[RM.o + i] + (long)Mt.val[RT.o + i];
+    bytes += 8 * (long)width;
     const long inc = wave_incl_scan(w);
     const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
     return f < 0 ? hi - 1 : lo + f;
@@ -506,8 +554,7 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
     } else {
       for (int32_t i = lane; i < width; i += kWave) scr[i] = 0;
       wave_fence();
-      gather_row(M, RM, lo, hi, scr, bytes);
-      gather_row(Mt, RT, lo, hi, scr, bytes);
+      gather_rows(M, RM, Mt, RT, lo, hi, scr, bytes);
     }
     wave_fence();
     if (lane == 0 && excl >= lo && excl < hi) scr[excl - lo] = 0;
@@ -696,10 +743,11 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   for (int i = lane; i < kNBK; i += kWave) m.bs[a * kNBK + i] = bk[i];
 }
 
-// Rows queued by k_apply (key count past 3/4 of the slab), handled between
-// two batches, zero counts dropped: a row whose live keys fit half its slab
-// (and at most kCompact slots) is compacted in place through LDS; any other
-// row moves to a slab >= 4*live+16 (direct once that reaches B/8) at the free
+// Rows queued by k_apply (key count past 1/4 of the slab, or an increment
+// parked), handled between two batches, zero counts dropped: a row whose live
+// keys and parked increments fit an eighth of its slab (and at most kCompact
+// slots) is compacted in place through LDS; any other
+// row moves to a slab >= 8*live+16 (direct once that reaches B/8) at the free
 // tail of the pool.  Bucket sums are unchanged.  One launch serves M and M^T.
 constexpr int kCompact = 8192;
 constexpr int32_t kOvfCap = 1 << 22;  // parked increments per orientation
@@ -748,7 +796,7 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     __syncthreads();
     const int32_t live = s_live;
     const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
-    if (!dold && 2 * (live + pend) <= c && c <= kCompact) {
+    if (!dold && 8 * (live + pend) <= c && c <= kCompact) {
       // in place: gather the live entries, clear, reinsert
       __syncthreads();
       if (threadIdx.x == 0) s_live = 0;
@@ -788,7 +836,7 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     }
     if (threadIdx.x == 0) {
       int64_t nc = 32;
-      while (nc < 4 * ((int64_t)live + pend) + 16) nc <<= 1;
+      while (nc < 8 * ((int64_t)live + pend) + 16) nc <<= 1;
       if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
       if (8 * nc >= B) nc = B;
       const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
@@ -1102,7 +1150,8 @@ struct NodalArgs {
   int32_t B;
   int32_t beg, end;
   int32_t exact;
-  uint64_t seed, stream;
+  int32_t bidx;  // batch index in the sweep: random stream sweep->stream0 + bidx
+  uint64_t seed;
   const int64_t* big_off;
   int32_t* big_tab;
   int32_t* prop_s;
@@ -1141,11 +1190,16 @@ struct Table {
   int32_t cap;
 };
 
+// kBig selects the node's global table (more than kTabMaxDeg list entries)
+// or the wavefront's LDS table at compile time, so the LDS form compiles to
+// LDS instructions (a pointer that may be either would make every table
+// access a flat access that also waits for the wavefront's global loads).
+template <bool kBig>
 __device__ __forceinline__ Table tab_open(int32_t* lds, const int64_t* big_off, int32_t* big_tab,
                                           int32_t n, int64_t kc) {
   const int lane = lane_id();
   Table T;
-  if (kc > kTabMaxDeg) {
+  if (kBig) {
     int32_t cap = 1;
     while (cap < 2 * kc) cap <<= 1;
     T.cap = cap;
@@ -1237,7 +1291,7 @@ __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, 
   const Probe pSR = m_probe(a.M, sM, r), pSS = m_probe(a.M, sM, s);
   // changed entries of rows r, s (out-edges) and columns r, s (in-edges)
   i128 acc = 0;
-  long Wo = 0, Wi = 0, lookups = 0;
+  long lookups = 0;
   bool neg = false;
   for (int32_t j = lane; j < nent; j += kWave) {
     const int32_t b = T.tk[j], wo = T.two[j], wi = T.twi[j];
@@ -1255,25 +1309,26 @@ __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, 
       lookups += 2;
       neg |= Rr - wo < 0;
       acc += Lterm(lam, Rr) + Lterm(lam, Rs) - Lterm(lam, Rr - wo) - Lterm(lam, Rs + wo);
-      Wo += wo;
     }
     if (wi > 0) {
       const int64_t Cr = m_value(a.Mt, rT, b, p2), Cs = m_value(a.Mt, sT, b, p3);
       lookups += 2;
       neg |= Cr - wi < 0;
       acc += Lterm(lam, Cr) + Lterm(lam, Cs) - Lterm(lam, Cr - wi) - Lterm(lam, Cs + wi);
-      Wi += wi;
     }
   }
   if (neg) atomicOr(&a.status->flags, kFlagNegative);
   acc = wave_sum128(acc);
-  Wo = wave_sum(Wo);
-  Wi = wave_sum(Wi);
-  wor = wave_sum(wor);
-  wos = wave_sum(wos);
-  wir = wave_sum(wir);
-  wis = wave_sum(wis);
-  wself = wave_sum(wself);
+  // blocks r and s own one table slot each, so one lane at most holds their
+  // weights; the other entries' weights are the rest of k_out / k_in
+  const unsigned long long hr = __ballot(wor != 0 || wir != 0);
+  const unsigned long long hs = __ballot(wos != 0 || wis != 0);
+  wor = hr ? read_lane(wor, __ffsll(hr) - 1) : 0;
+  wir = hr ? read_lane(wir, __ffsll(hr) - 1) : 0;
+  wos = hs ? read_lane(wos, __ffsll(hs) - 1) : 0;
+  wis = hs ? read_lane(wis, __ffsll(hs) - 1) : 0;
+  const long Wo = k_out - wor - wos, Wi = k_in - wir - wis;
+  if (__ballot(wself != 0)) wself = wave_sum(wself);  // self-loops are rare
   // neighbour ids + labels (+ weights); rows (offset + capacity) and degrees
   // of r and s; 2 lookups (key + count) per changed entry and 4 corner lookups
   bytes += (a.g.out_w ? 12 : 8) * kc + 48 + 16 + 8 * (wave_sum(lookups) + 4);
@@ -1315,6 +1370,7 @@ __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, 
 // neighbours' labels with the drawn neighbour's; its block's degrees (and, when
 // the proposal follows M, its rows' bucket sums and one bucket); s's rows and
 // degrees; then every lookup of the dS at once.
+template <bool kBig>
 __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int32_t n, int32_t* tab,
                           int32_t* scr) {
   const int lane = lane_id();
@@ -1325,7 +1381,8 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
   const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
   const int64_t kc = kco + kci;
   uint64_t x[4];
-  usap::philox4x64((uint64_t)n, 0, a.stream, usap::kTagNodal, a.seed, 0, x);
+  usap::philox4x64((uint64_t)n, 0, a.sweep->stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0,
+                   x);
   const Row rM = m_row(a.M, r), rT = m_row(a.Mt, r);
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   int64_t k_out = kco, k_in = kci;
@@ -1339,7 +1396,7 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
   const int64_t k = k_out + k_in;
   // list offsets, own label, outputs (s, dS, inf, accept, r) (+ weights)
   long bytes = 32 + 4 + 32 + (g.out_w ? 4 * kc : 0);
-  const Table T = tab_open(tab, a.big_off, a.big_tab, n, kc);
+  const Table T = tab_open<kBig>(tab, a.big_off, a.big_tab, n, kc);
   const uint32_t mask = (uint32_t)T.cap - 1;
   // the drawn neighbour (graph_parallel.hpp:1010-1027): unit t of the list
   const int64_t t = k > 0 ? (int64_t)usap::idx_of(x[0], k) : 0;
@@ -1422,7 +1479,12 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
   long bytes = 0;
-  if (n64 < a.end && !a.sweep->stop) bytes = nodal_one(a, item, (int32_t)n64, s_tab[w], s_scr[w]);
+  if (n64 < a.end && !a.sweep->stop) {
+    const int32_t n = (int32_t)n64;
+    const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
+    bytes = kc > kTabMaxDeg ? nodal_one<true>(a, item, n, s_tab[w], s_scr[w])
+                            : nodal_one<false>(a, item, n, s_tab[w], s_scr[w]);
+  }
   __syncthreads();  // every wavefront is done with its scratch
   unsigned long long* s_bytes = reinterpret_cast<unsigned long long*>(&s_scr[0][0]);
   if (lane_id() == 0) s_bytes[w] = (unsigned long long)bytes;
@@ -1503,7 +1565,9 @@ __device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_
 // table entry are read together before the adds.  Old labels of nodes inside
 // the batch come from prop_r, so writing labels cannot race with their reads.
 // Returns the algorithmic bytes.
-__device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t r, int32_t s, int32_t* tab) {
+template <bool kBig>
+__device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t r, int32_t s,
+                                          int32_t* tab) {
   const int lane = lane_id();
   const GraphView& g = a.g;
   int32_t* flags = &a.status->flags;
@@ -1511,7 +1575,7 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
   const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
   const int64_t kc = kco + kci;
   const Row rM = m_row(a.M, r), rT = m_row(a.Mt, r), sM = m_row(a.M, s), sT = m_row(a.Mt, s);
-  const Table T = tab_open(tab, a.big_off, a.big_tab, n, kc);
+  const Table T = tab_open<kBig>(tab, a.big_off, a.big_tab, n, kc);
   const uint32_t mask = (uint32_t)T.cap - 1;
   long wo_sum = 0, wi_sum = 0;
   for (int64_t base = 0; base < kc; base += kEdgeRegs * kWave) {
@@ -1666,8 +1730,12 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
   long bytes = 0;
-  if (n64 < a.end && a.prop_acc[item])
-    bytes = apply_one(a, (int32_t)n64, a.prop_r[item], a.prop_s[item], s_tab[w]);
+  if (n64 < a.end && a.prop_acc[item]) {
+    const int32_t n = (int32_t)n64;
+    const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
+    bytes = kc > kTabMaxDeg ? apply_one<true>(a, n, a.prop_r[item], a.prop_s[item], s_tab[w])
+                            : apply_one<false>(a, n, a.prop_r[item], a.prop_s[item], s_tab[w]);
+  }
   if (lane_id() == 0) s_ab[w] = (unsigned long long)bytes;
   __syncthreads();
   if (threadIdx.x == 0) {  // this block's share of the batch sums
@@ -1771,6 +1839,12 @@ struct usap_engine {
   Sweep* h_sweep = nullptr;   // pinned host copy
   int32_t* prop_r = nullptr;  // old label of each batch node
   std::vector<hipEvent_t> bev;  // per-batch kernel events (profiling)
+  // captured sweep (usap_params.graphs): replayed while its key matches
+  hipGraphExec_t gexec = nullptr;
+  int32_t g_B = -1, g_batch = -1, g_nb = -1;
+  uint64_t g_epoch = 0;
+  uint64_t epoch = 0;  // bumped by every device allocation (pointers may change)
+  int64_t graph_captures = 0;
   int32_t* build_flags = nullptr;
   int64_t* big_off = nullptr;
   int32_t* big_tab = nullptr;
@@ -1822,6 +1896,7 @@ template <typename T>
 int dalloc(usap_engine* e, T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
+  e->epoch++;
   hipError_t st = hipMallocAsync((void**)p, n * sizeof(T), e->stream);
   if (st != hipSuccess) {
     *p = nullptr;
@@ -2033,6 +2108,28 @@ int check_engine(usap_engine* e) {
 }  // namespace
 
 namespace usap {
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+thread_local std::string t_err;
+}  // namespace
+
+void set_detached_error(const std::string& msg) {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  if (g_err.empty()) g_err = msg;  // the first failure is the informative one
+}
+
+void clear_detached_error() {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  g_err.clear();
+}
+
+const char* detached_error() {
+  std::lock_guard<std::mutex> lk(g_err_mu);
+  t_err = g_err.empty() ? "no error recorded" : g_err;
+  return t_err.c_str();
+}
+
 std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log; }
 const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
@@ -2078,9 +2175,13 @@ void usap_default_params(usap_params* p) {
   p->device = 0;
   p->verbose = 0;
   p->slab_slack = 4;
+  p->graphs = 0;
 }
 
-const char* usap_last_error(const usap_engine* e) { return e ? e->err.c_str() : "null engine"; }
+const char* usap_last_error(const usap_engine* e) {
+  if (e) return e->err.c_str();
+  return usap::detached_error();
+}
 
 int usap_destroy(usap_engine* e) {
   if (!e) return USAP_OK;
@@ -2130,6 +2231,7 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->prop_r);
   dfree(e, e->build_flags);
   for (hipEvent_t ev : e->bev) (void)hipEventDestroy(ev);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   pin_put(e->h_sweep, sizeof(Sweep));
   dfree(e, e->big_off);
   dfree(e, e->big_tab);
@@ -2599,8 +2701,20 @@ int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new) {
 
 namespace {
 
-// Launches k_nodal + k_apply for nodes [beg, end) with random stream `stream`.
-int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size_t ev_slot) {
+// Adds the kernel times of batch slot i (events of launch_batch) to the profile.
+void prof_batch(usap_engine* e, size_t i) {
+  const int kid[3] = {USAP_K_NODAL, USAP_K_APPLY, USAP_K_GROW};
+  for (int k = 0; k < 3; ++k) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e->bev[6 * i + 2 * k], e->bev[6 * i + 2 * k + 1]);
+    e->kp[kid[k]].ms += ms;
+    e->kp[kid[k]].launches += 1;
+  }
+}
+
+// Launches k_nodal + k_apply + k_grow for nodes [beg, end), batch bidx of the
+// sweep (random stream sweep->stream0 + bidx).
+int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t ev_slot) {
   const int32_t B = e->B;
   hipStream_t st = e->stream;
   NodalArgs a;
@@ -2616,7 +2730,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   a.end = end;
   a.exact = e->p.ds_mode == USAP_DS_EXACT;
   a.seed = e->p.seed;
-  a.stream = stream;
+  a.bidx = bidx;
   a.big_off = e->big_off;
   a.big_tab = e->big_tab;
   a.prop_s = e->prop_s;
@@ -2650,17 +2764,6 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   b.status = e->status;
   b.sweep = e->sweep;
   const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
-  if (e->prof) {
-    while (e->bev.size() < 3 * (ev_slot + 1)) {
-      hipEvent_t ev;
-      HIPCHK(hipEventCreate(&ev));
-      e->bev.push_back(ev);
-    }
-    HIPCHK(hipEventRecord(e->bev[3 * ev_slot], st));
-  }
-  k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
-  if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 1], st));
-  k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
   GrowArgs ga;
   DevSpM* ms[2] = {&e->M, &e->Mt};
   for (int i = 0; i < 2; ++i) {
@@ -2680,41 +2783,93 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, uint64_t stream, size
   }
   ga.B = B;
   ga.sweep = e->sweep;
-  k_grow<<<128, 256, 0, st>>>(ga);
-  if (e->prof) HIPCHK(hipEventRecord(e->bev[3 * ev_slot + 2], st));
+  if (e->prof) {
+    // start/stop events attached to each launch time the kernel itself
+    while (e->bev.size() < 6 * (ev_slot + 1)) {
+      hipEvent_t ev;
+      HIPCHK(hipEventCreate(&ev));
+      e->bev.push_back(ev);
+    }
+    hipEvent_t* ev = &e->bev[6 * ev_slot];
+    hipExtLaunchKernelGGL(k_nodal, dim3(grid), dim3(kWave * kNodalWaves), 0, st, ev[0], ev[1], 0, a);
+    hipExtLaunchKernelGGL(k_apply, dim3(grid), dim3(kWave * kNodalWaves), 0, st, ev[2], ev[3], 0, b);
+    hipExtLaunchKernelGGL(k_grow, dim3(128), dim3(256), 0, st, ev[4], ev[5], 0, ga);
+  } else {
+    k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
+    k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
+    k_grow<<<128, 256, 0, st>>>(ga);
+  }
   HIPCHK(hipGetLastError());
   return USAP_OK;
 }
 
-// Runs the batches [first*batch, N) of a sweep from the device state already
-// in e->sweep, one host synchronisation at the end; rebuilds M and resumes
-// when a batch overflowed a row slab.  Stops early when the sweep's
-// convergence test fires.
-int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, uint64_t stream0,
-              int32_t* rebuilds) {
+// The batches [0, nb) of a sweep as one HIP graph, captured on first use
+// and replayed while B, the batch size and every device pointer the batch
+// kernels take (epoch) stay the same; the sweep's state (random stream,
+// convergence inputs) lives in device memory, so the kernels' arguments do
+// not change between sweeps.  The graph holds kernel nodes only.
+int sweep_graph(usap_engine* e, int32_t batch, int32_t nb) {
+  hipStream_t st = e->stream;
+  if (!e->gexec || e->g_B != e->B || e->g_batch != batch || e->g_nb != nb ||
+      e->g_epoch != e->epoch) {
+    if (e->gexec) {
+      (void)hipGraphExecDestroy(e->gexec);
+      e->gexec = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = USAP_OK;
+    for (int32_t i = 0; i < nb && rc == USAP_OK; ++i) {
+      const int32_t beg = i * batch;
+      const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
+      rc = launch_batch(e, beg, end, i, 0);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (rc != USAP_OK) {
+      if (ce == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(ce);
+    const hipError_t ie = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      e->gexec = nullptr;
+      return set_err(e, USAP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    e->g_B = e->B;
+    e->g_batch = batch;
+    e->g_nb = nb;
+    e->g_epoch = e->epoch;
+    e->graph_captures++;
+  }
+  HIPCHK(hipGraphLaunch(e->gexec, st));
+  return USAP_OK;
+}
+
+// Runs the batches [first, last) of a sweep (batch nodes each) from the
+// device state already in e->sweep, one host synchronisation at the end;
+// rebuilds M and resumes when a batch overflowed a row slab.  Stops early
+// when the sweep's convergence test fires.
+int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, int32_t* rebuilds) {
   hipStream_t st = e->stream;
   Sweep& hs = *e->h_sweep;
   int32_t next = first;
   for (;;) {
-    size_t slot = 0;
-    for (int32_t i = next; i < last; ++i, ++slot) {
-      const int32_t beg = i * batch;
-      const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
-      if (int rc = launch_batch(e, beg, end, stream0 + (uint64_t)i, slot)) return rc;
+    if (e->p.graphs && !e->prof && next == 0) {
+      if (int rc = sweep_graph(e, batch, last)) return rc;
+    } else {
+      size_t slot = 0;
+      for (int32_t i = next; i < last; ++i, ++slot) {
+        const int32_t beg = i * batch;
+        const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
+        if (int rc = launch_batch(e, beg, end, i, slot)) return rc;
+      }
     }
     HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const int32_t executed = first + hs.done - next;  // batches run in this launch round
     if (e->prof) {
-      for (int32_t i = 0; i < executed; ++i) {
-        float a = 0.f, b = 0.f;
-        (void)hipEventElapsedTime(&a, e->bev[3 * i], e->bev[3 * i + 1]);
-        (void)hipEventElapsedTime(&b, e->bev[3 * i + 1], e->bev[3 * i + 2]);
-        e->kp[USAP_K_NODAL].ms += a;
-        e->kp[USAP_K_NODAL].launches += 1;
-        e->kp[USAP_K_APPLY].ms += b;
-        e->kp[USAP_K_APPLY].launches += 1;
-      }
+      for (int32_t i = 0; i < executed; ++i) prof_batch(e, i);
     }
     if (hs.stop == 3) return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
     if (hs.flags & (kFlagLoad | kFlagFull)) {
@@ -2735,9 +2890,11 @@ int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, uint64
   return USAP_OK;
 }
 
-void init_sweep(usap_engine* e, double S, double mean, double thr, int32_t check) {
+void init_sweep(usap_engine* e, uint64_t stream0, double S, double mean, double thr,
+                int32_t check) {
   Sweep& hs = *e->h_sweep;
   std::memset(&hs, 0, sizeof(Sweep));
+  hs.stream0 = stream0;
   hs.S = S;
   hs.mean = mean;
   hs.thr = thr;
@@ -2754,21 +2911,19 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
   if (begin < 0 || end > e->N || begin > end) return USAP_EINVAL;
   e->last_beg = begin;
   e->last_end = end;
-  init_sweep(e, 0.0, 0.0, 0.0, 0);
+  init_sweep(e, stream, 0.0, 0.0, 0.0, 0);
   if (begin < end) {
     HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
     // one batch of size end-begin starting at begin: index 0 of a sweep whose
     // "batch" starts at begin
-    if (int rc = launch_batch(e, begin, end, stream, 0)) return rc;
+    if (int rc = launch_batch(e, begin, end, 0, 0)) return rc;
     Sweep& hs = *e->h_sweep;
     HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) {
-      float a = 0.f, b = 0.f;
-      (void)hipEventElapsedTime(&a, e->bev[0], e->bev[1]);
-      (void)hipEventElapsedTime(&b, e->bev[1], e->bev[2]);
-      prof_add(e, USAP_K_NODAL, a, 1, (double)hs.nodal_bytes);
-      prof_add(e, USAP_K_APPLY, b, 1, (double)hs.apply_bytes);
+      prof_batch(e, 0);
+      e->kp[USAP_K_NODAL].bytes += (double)hs.nodal_bytes;
+      e->kp[USAP_K_APPLY].bytes += (double)hs.apply_bytes;
     }
     if (hs.stop == 3) return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
     if (hs.flags & (kFlagLoad | kFlagFull))
@@ -2785,10 +2940,10 @@ int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, 
   if (int rc = check_engine(e)) return rc;
   if (batch < 1 || !out) return USAP_EINVAL;
   const int32_t nb = (int32_t)(((int64_t)e->N + batch - 1) / batch);
-  init_sweep(e, S, mean, thr, check);
+  init_sweep(e, stream0, S, mean, thr, check);
   HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
   int32_t rebuilds = 0;
-  if (int rc = run_sweep(e, batch, 0, nb, stream0, &rebuilds)) return rc;
+  if (int rc = run_sweep(e, batch, 0, nb, &rebuilds)) return rc;
   const Sweep& hs = *e->h_sweep;
   const int32_t last = hs.done - 1;  // the last executed batch
   e->last_beg = last * batch;

@@ -1,6 +1,7 @@
 // engine_internal.hpp -- engine accessors used by the host driver inside the
 // library (not part of the public C-ABI).
 #pragma once
+#include <string>
 #include <vector>
 
 #include "usap.h"
@@ -18,4 +19,9 @@ int64_t engine_grows(usap_engine* e);
 // insertions, pool exhaustions.
 void engine_grow_debug(usap_engine* e, long long out[4]);
 void engine_set_error(usap_engine* e, const char* msg);
+// Error of a call that has no engine handle to hold it (batch, serve): the
+// first one recorded since the last clear; usap_last_error(NULL) returns it.
+void set_detached_error(const std::string& msg);
+void clear_detached_error();
+const char* detached_error();
 }  // namespace usap

@@ -24,9 +24,11 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
+#include "engine_internal.hpp"
 #include "usap.h"
 
 namespace {
@@ -181,6 +183,10 @@ struct Server {
       rec.t_start = t;
       lk.unlock();
       int rc = run_stage(g, k, rec);
+      if (rc != USAP_OK)
+        usap::set_detached_error("usap_serve: graph " + std::to_string(g) + " stage " +
+                                 std::to_string(k) + ": error " + std::to_string(rc) + ": " +
+                                 (s.eng ? usap_last_error(s.eng) : "usap_create failed"));
       lk.lock();
       rec.t_done = now();
       rec.error = rc;
@@ -234,6 +240,7 @@ int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts*
     if (graphs[i].N < 1 || graphs[i].E < 0 || (graphs[i].E > 0 && (!graphs[i].src || !graphs[i].dst)) ||
         graphs[i].arrival_s < 0)
       return USAP_EINVAL;
+  usap::clear_detached_error();
   Server sv;
   sv.n = n;
   sv.graphs = graphs;

@@ -6,8 +6,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (UsapParams, UsapServeGraph, UsapServeOpts, UsapServeStage, UsapServeStats,
-                   check, lib)
+from ._lib import (UsapError, UsapParams, UsapServeGraph, UsapServeOpts, UsapServeStage,
+                   UsapServeStats, lib)
 from .engine import default_params
 from .graph import generate_dcsbm
 
@@ -75,9 +75,10 @@ def serve(graphs, arrivals, params, stages=10, interval_s=0.0, workers=4, hbm_bu
     rc = lib().usap_serve(n, arr, C.byref(opt), labels.ctypes.data, off.ctypes.data, recs,
                           C.byref(st))
     stats = st.as_dict()
-    if rc != 0 and stats["failed"] == 0:
-        check(rc)
     stats["rc"] = rc
+    stats["error"] = lib().usap_last_error(None).decode() if rc != 0 else ""
+    if rc != 0 and stats["failed"] == 0:
+        raise UsapError(rc, stats["error"])
     per = [[recs[i * stages + k].as_dict() for k in range(stages)] for i in range(n)]
     return [labels[off[i]:off[i + 1]].copy() for i in range(n)], per, stats
 
