@@ -72,3 +72,19 @@ def test_batch_autotune(usap):
     assert all(r > 0 for _, r in trials)
     labs, sts, bs = usap.partition_batch(graphs, params, workers=0)
     assert bs["failed"] == 0 and 1 <= bs["workers"] <= 16
+
+
+@pytest.mark.gpu
+def test_batch_with_graph_replay(usap):
+    """Several engines in flight, each replaying its sweeps from a captured
+    HIP graph: same partitions as alone without graphs."""
+    graphs, specs = usap.batch_graphs(n=10, seed=12, n_min=300, n_max=3000)
+    params = [dict(seed=s["seed"], graphs=1) for s in specs]
+    labs, sts, bs = usap.partition_batch(graphs, params, workers=6)
+    assert bs["failed"] == 0, bs["error"]
+    for g, p, lb, sb in zip(graphs, params, labs, sts):
+        eng = usap.Engine(g, **dict(p, graphs=0))
+        lab, st = eng.partition()
+        eng.close()
+        np.testing.assert_array_equal(lab, lb)
+        assert st["S"] == sb["S"]

@@ -6,8 +6,9 @@
 //   row a occupies slots [off[a], off[a]+cap[a]);
 //   cap[a] == B  -> direct-mapped row (slot = column), used when a hashed row
 //                   would need >= B slots (dense regime, small B);
-//   cap[a] <  B  -> open-addressing hash (power of two, linear probing),
-//                   keys int32 (-1 empty), counts int32.
+//   cap[a] <  B  -> open-addressing hash (power of two, linear probing);
+//                   a slot is (key, count), int32 each (key -1 = empty), so a
+//                   probe and the count update after it share a cache line.
 // Keys are never deleted inside a sweep (a count may drop to 0).  Each row
 // also keeps kNBK bucket sums (sum of its counts per key range) for the
 // two-level inverse-CDF pick.  A row whose key count passes 1/4 of its slab
@@ -115,13 +116,19 @@ struct GraphView {
   const int32_t* in_w;
 };
 
+// One slot of a row slab: key (-1 empty; unused in a direct row) and count
+// side by side, so a probe and the count update that follows it touch one
+// cache line.
+struct __align__(8) Slot {
+  int32_t k, v;
+};
+
 // One orientation of M, as seen by kernels.
 struct MView {
   const int64_t* off;
   const int32_t* cap;
   int32_t* nkeys;
-  int32_t* key;
-  int32_t* val;
+  Slot* kv;        // row slabs
   int32_t* bs;     // [B][kNBK]: sum of the row's counts per key bucket
   int32_t* gmark;  // [B]: row queued for growth
   int32_t* gq;     // [B]: growth queue
@@ -180,19 +187,21 @@ struct Probe {
 __device__ __forceinline__ Probe m_probe(const MView& m, Row R, int32_t b) {
   Probe p;
   p.slot = m_slot(m, R, b);
-  p.k = m.key[p.slot];
-  p.v = m.val[p.slot];
+  const Slot q = m.kv[p.slot];
+  p.k = q.k;
+  p.v = q.v;
   return p;
 }
 
 // Linear probing on from slot `slot` of a hashed row after a collision there.
-__device__ __forceinline__ int32_t m_value_more(const int32_t* key, const int32_t* val, int64_t o,
+__device__ __forceinline__ int32_t m_value_more(const Slot* kv, int64_t o,
                                              int32_t c, int32_t b, int64_t slot) {
   const uint32_t mask = (uint32_t)c - 1;
   uint32_t h = (uint32_t)(slot - o);
   for (int32_t probe = 1; probe < c; ++probe) {
     h = (h + 1) & mask;
-    const int32_t k = key[o + h], v = val[o + h];
+    const Slot q = kv[o + h];
+    const int32_t k = q.k, v = q.v;
     if (k == b) return v;
     if (k < 0) return 0;
   }
@@ -203,7 +212,7 @@ __device__ __forceinline__ int32_t m_value_more(const int32_t* key, const int32_
 __device__ __forceinline__ int32_t m_value(const MView& m, Row R, int32_t b, const Probe& p) {
   if (R.c == m.B || p.k == b) return p.v;
   if (p.k < 0) return 0;
-  return m_value_more(m.key, m.val, R.o, R.c, b, p.slot);
+  return m_value_more(m.kv, R.o, R.c, b, p.slot);
 }
 
 // M[a][b]
@@ -222,7 +231,7 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
                                          int64_t slot, int32_t k, int32_t* flags) {
   if (kBuckets) atomicAdd(&m.bs[(int64_t)a * kNBK + (b >> m.bshift)], w);
   if (R.c == m.B || k == b) {
-    atomicAdd(&m.val[slot], w);
+    atomicAdd(&m.kv[slot].v, w);
     return;
   }
   const int64_t o = R.o;
@@ -230,13 +239,13 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
   const uint32_t mask = (uint32_t)c - 1;
   uint32_t h = (uint32_t)(slot - o);
   for (int32_t probe = 0; probe < c; ++probe) {
-    if (probe > 0) k = ld_relaxed(&m.key[o + h]);
+    if (probe > 0) k = ld_relaxed(&m.kv[o + h].k);
     if (k < 0) {
       if (w < 0) {
         atomicOr(flags, kFlagMissing);
         return;
       }
-      k = atomicCAS(&m.key[o + h], -1, b);
+      k = atomicCAS(&m.kv[o + h].k, -1, b);
       if (k == -1) {
         const int32_t n = atomicAdd(&m.nkeys[a], 1) + 1;
         // past 1/4 load during nodal batches (short probe chains for the
@@ -252,7 +261,7 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
       }
     }
     if (k == b) {
-      atomicAdd(&m.val[o + h], w);
+      atomicAdd(&m.kv[o + h].v, w);
       return;
     }
     h = (h + 1) & mask;
@@ -275,7 +284,7 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
 // First key of b in row R, for m_add_at.
 __device__ __forceinline__ int32_t m_first_key(const MView& m, Row R, int32_t b, int64_t& slot) {
   slot = m_slot(m, R, b);
-  return R.c == m.B ? b : ld_relaxed(&m.key[slot]);
+  return R.c == m.B ? b : ld_relaxed(&m.kv[slot].k);
 }
 
 // M[a][b] += w
@@ -370,11 +379,11 @@ __device__ __forceinline__ long count_range_lane(const MView& m, int64_t o, int3
   long acc = 0;
   if (c == m.B) {
     for (int32_t i = lo + lane; i <= x; i += kWave)
-      if (i != excl) acc += m.val[o + i];
+      if (i != excl) acc += m.kv[o + i].v;
   } else {
     for (int32_t j = lane; j < c; j += kWave) {
-      const int32_t k = m.key[o + j];
-      if (k >= lo && k <= x && k != excl) acc += m.val[o + j];
+      const int32_t k = m.kv[o + j].k;
+      if (k >= lo && k <= x && k != excl) acc += m.kv[o + j].v;
     }
   }
   return acc;
@@ -387,7 +396,7 @@ __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, in
                                            int32_t* scr, long& bytes) {
   const int lane = lane_id();
   if (R.c == m.B) {
-    for (int32_t i = lane; i < hi - lo; i += kWave) atomicAdd(&scr[i], m.val[R.o + lo + i]);
+    for (int32_t i = lane; i < hi - lo; i += kWave) atomicAdd(&scr[i], m.kv[R.o + lo + i].v);
     bytes += 4 * (long)(hi - lo);
     return;
   }
@@ -399,8 +408,9 @@ __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, in
       k[q] = -1;
       v[q] = 0;
       if (j < R.c) {
-        k[q] = m.key[R.o + j];
-        v[q] = m.val[R.o + j];
+        const Slot x = m.kv[R.o + j];
+        k[q] = x.k;
+        v[q] = x.v;
       }
     }
 #pragma unroll
@@ -427,14 +437,14 @@ __device__ __forceinline__ void gather_rows(const MView& M, Row RM, const MView&
     for (int q = 0; q < 8; ++q) {
       const bool t = q >= 4;
       const Row& R = t ? RT : RM;
-      const int32_t* key = t ? Mt.key : M.key;
-      const int32_t* val = t ? Mt.val : M.val;
+      const Slot* kv = t ? Mt.kv : M.kv;
       const int32_t j = base + (q & 3) * kWave + lane;
       k[q] = -1;
       v[q] = 0;
       if (j < R.c) {
-        k[q] = key[R.o + j];
-        v[q] = val[R.o + j];
+        const Slot x = kv[R.o + j];
+        k[q] = x.k;
+        v[q] = x.v;
       }
     }
 #pragma unroll
@@ -529,7 +539,7 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
   if (width <= kWave && RM.c == B && RT.c == B) {  // direct rows: one load per key
     long w = 0;
     const int32_t i = lo + lane;
-    if (lane < width && i != excl) w = (long)M.val[RM.o + i] + (long)Mt.val[RT.o + i];
+    if (lane < width && i != excl) w = (long)M.kv[RM.o + i].v + (long)Mt.kv[RT.o + i].v;
     bytes += 8 * (long)width;
     const long inc = wave_incl_scan(w);
     const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
@@ -542,7 +552,7 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int32_t i = base + q * kWave + lane;
-          v[q] = i < width ? M.val[RM.o + lo + i] + Mt.val[RT.o + lo + i] : 0;
+          v[q] = i < width ? M.kv[RM.o + lo + i].v + Mt.kv[RT.o + lo + i].v : 0;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -676,11 +686,10 @@ __global__ void k_caps(int32_t B, int32_t slack, const int32_t* ecnt_out, const 
   tmpT[a] = ct;
 }
 
-__global__ void k_fill(int32_t* key, int32_t* val, int64_t n) {
+__global__ void k_fill(Slot* kv, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    key[i] = -1;
-    val[i] = 0;
+    kv[i] = Slot{-1, 0};
   }
 }
 
@@ -717,8 +726,8 @@ __global__ __launch_bounds__(256) void k_insert_small(int64_t E, const int32_t* 
     const int32_t v = s_h[i];
     if (!v) continue;
     const int32_t a = i / B, b = i - a * B;
-    atomicAdd(&M.val[M.off[a] + b], v);
-    atomicAdd(&Mt.val[Mt.off[b] + a], v);
+    atomicAdd(&M.kv[M.off[a] + b].v, v);
+    atomicAdd(&Mt.kv[Mt.off[b] + a].v, v);
   }
 }
 
@@ -735,8 +744,8 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   const int32_t c = m.cap[a];
   const bool direct = c == m.B;
   for (int32_t j = lane; j < c; j += kWave) {
-    const int32_t k = direct ? j : m.key[o + j];
-    const int32_t v = m.val[o + j];
+    const int32_t k = direct ? j : m.kv[o + j].k;
+    const int32_t v = m.kv[o + j].v;
     if (k >= 0 && v != 0) atomicAdd(&bk[k >> m.bshift], v);
   }
   wave_fence();
@@ -756,8 +765,7 @@ struct GrowSide {
   int64_t* off;
   int32_t* cap;
   int32_t* nkeys;
-  int32_t* key;
-  int32_t* val;
+  Slot* kv;
   int32_t* gmark;
   const int32_t* gq;
   int32_t* gpend;
@@ -790,8 +798,8 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     if (threadIdx.x == 0) s_live = 0;
     __syncthreads();
     for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-      const int32_t k = dold ? j : g.key[o + j];
-      if (k >= 0 && g.val[o + j] != 0) atomicAdd(&s_live, 1);
+      const int32_t k = dold ? j : g.kv[o + j].k;
+      if (k >= 0 && g.kv[o + j].v != 0) atomicAdd(&s_live, 1);
     }
     __syncthreads();
     const int32_t live = s_live;
@@ -802,7 +810,8 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       if (threadIdx.x == 0) s_live = 0;
       __syncthreads();
       for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        const int32_t k = g.key[o + j], v = g.val[o + j];
+        const Slot x = g.kv[o + j];
+        const int32_t k = x.k, v = x.v;
         if (k >= 0 && v != 0) {
           const int32_t i = atomicAdd(&s_live, 1);
           s_k[i] = k;
@@ -811,16 +820,15 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       }
       __syncthreads();
       for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        g.key[o + j] = -1;
-        g.val[o + j] = 0;
+        g.kv[o + j] = Slot{-1, 0};
       }
       __threadfence_block();
       __syncthreads();
       const uint32_t mask = (uint32_t)c - 1;
       for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
         uint32_t h = hash32(s_k[i]) & mask;
-        while (atomicCAS(&g.key[o + h], -1, s_k[i]) != -1) h = (h + 1) & mask;
-        g.val[o + h] = s_v[i];
+        while (atomicCAS(&g.kv[o + h].k, -1, s_k[i]) != -1) h = (h + 1) & mask;
+        g.kv[o + h].v = s_v[i];
       }
       __threadfence_block();
       __syncthreads();
@@ -859,25 +867,24 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     if (nc > 0) {
       const int64_t nb = s_base;
       for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) {
-        g.key[nb + i] = -1;
-        g.val[nb + i] = 0;
+        g.kv[nb + i] = Slot{-1, 0};
       }
       __threadfence_block();
       __syncthreads();
       const bool dnew = nc == B;
       const uint32_t mask = (uint32_t)nc - 1;
       for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        const int32_t k = g.key[o + j];
-        const int32_t v = g.val[o + j];
+        const Slot x = g.kv[o + j];
+        const int32_t k = x.k, v = x.v;
         if (k < 0 || v == 0) continue;
-        g.val[o + j] = 0;  // the old slab stays in the pool as zeros until a rebuild
+        g.kv[o + j].v = 0;  // the old slab stays in the pool as zeros until a rebuild
         if (dnew) {
-          g.val[nb + k] = v;
+          g.kv[nb + k].v = v;
           continue;
         }
         uint32_t h = hash32(k) & mask;
-        while (atomicCAS(&g.key[nb + h], -1, k) != -1) h = (h + 1) & mask;
-        g.val[nb + h] = v;
+        while (atomicCAS(&g.kv[nb + h].k, -1, k) != -1) h = (h + 1) & mask;
+        g.kv[nb + h].v = v;
       }
       __threadfence_block();
       __syncthreads();
@@ -913,23 +920,23 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       const int64_t o = g.off[a];
       const int32_t c = g.cap[a];
       if (c == B) {
-        atomicAdd(&g.val[o + b], w);
+        atomicAdd(&g.kv[o + b].v, w);
         continue;
       }
       const uint32_t mask = (uint32_t)c - 1;
       uint32_t h = hash32(b) & mask;
       bool done = false;
       for (int32_t probe = 0; probe < c && !done; ++probe) {
-        int32_t k = __hip_atomic_load(&g.key[o + h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t k = __hip_atomic_load(&g.kv[o + h].k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (k < 0) {
-          k = atomicCAS(&g.key[o + h], -1, b);
+          k = atomicCAS(&g.kv[o + h].k, -1, b);
           if (k == -1) {
             atomicAdd(&g.nkeys[a], 1);
             k = b;
           }
         }
         if (k == b) {
-          atomicAdd(&g.val[o + h], w);
+          atomicAdd(&g.kv[o + h].v, w);
           done = true;
         }
         h = (h + 1) & mask;
@@ -974,7 +981,7 @@ __device__ void block_store128(i128 v, long* out) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_entropy(const int32_t* __restrict__ val,
+__global__ __launch_bounds__(256) void k_entropy(const Slot* __restrict__ kv,
                                                  const GrowState* gst,
                                                  const int32_t* __restrict__ d_out,
                                                  const int32_t* __restrict__ d_in, int32_t B,
@@ -985,7 +992,7 @@ __global__ __launch_bounds__(256) void k_entropy(const int32_t* __restrict__ val
   i128 acc = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
-    const int32_t v = val[i];
+    const int32_t v = kv[i].v;
     if (v > 0) acc += (i128)v * (i128)lam[v];
   }
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += stride)
@@ -1022,8 +1029,8 @@ __device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r
   const bool direct = c == m.B;
   i128 acc = 0;
   for (int32_t j = lane; j < c; j += kWave) {
-    const int32_t i = direct ? j : m.key[o + j];
-    const int32_t m1 = m.val[o + j];
+    const int32_t i = direct ? j : m.kv[o + j].k;
+    const int32_t m1 = m.kv[o + j].v;
     if (i < 0 || m1 <= 0 || i == r || i == s) continue;
     const int32_t m2 = m_get(m, a2, i);
     if (m2 > 0) acc += Lterm(lam, m1) + Lterm(lam, m2) - Lterm(lam, (int64_t)m1 + m2);
@@ -1779,8 +1786,7 @@ struct DevSpM {
   int64_t* off = nullptr;
   int32_t* cap = nullptr;
   int32_t* nkeys = nullptr;
-  int32_t* key = nullptr;
-  int32_t* val = nullptr;
+  Slot* kv = nullptr;     // row slabs [P]
   int32_t* bs = nullptr;  // [bs_rows][kNBK]
   int32_t bs_rows = 0;
   int32_t* gmark = nullptr;  // [N]
@@ -1950,8 +1956,7 @@ MView view(const DevSpM& m, int32_t B) {
   v.off = m.off;
   v.cap = m.cap;
   v.nkeys = m.nkeys;
-  v.key = m.key;
-  v.val = m.val;
+  v.kv = m.kv;
   v.bs = m.bs;
   v.gmark = m.gmark;
   v.gq = m.gq;
@@ -2046,16 +2051,14 @@ int rebuild_M(usap_engine* e) {
   for (int i = 0; i < 2; ++i) {
     DevSpM& m = *ms[i];
     m.total = tot[i];
-    if (2 * m.total > m.P) {  // half the pool is headroom for rows that grow
-      dfree(e, m.key);
-      dfree(e, m.val);
-      const int64_t P = 2 * m.total + (int64_t)4 * B + 4096;
-      if (int rc = dalloc(e, &m.key, (size_t)P)) return rc;
-      if (int rc = dalloc(e, &m.val, (size_t)P)) return rc;
+    if (3 * m.total > m.P) {  // two thirds of the pool is headroom for rows that grow
+      dfree(e, m.kv);
+      const int64_t P = 3 * m.total + (int64_t)4 * B + 4096;
+      if (int rc = dalloc(e, &m.kv, (size_t)P)) return rc;
       m.P = P;
     }
     if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
-        m.key, m.val, m.total);
+        m.kv, m.total);
     HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
     HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
     HIPCHK(hipMemsetAsync(m.gpend, 0, sizeof(int32_t) * B, st));
@@ -2134,25 +2137,31 @@ std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log;
 const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
 int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
+// Growth counters of both orientations, read on the engine's own stream
+// (never the legacy null stream, which other engines' threads must not see).
+static bool read_grow_states(usap_engine* e, GrowState g[2]) {
+  DevSpM* ms[2] = {&e->M, &e->Mt};
+  for (int i = 0; i < 2; ++i)
+    if (hipMemcpyAsync(&g[i], ms[i]->gst, sizeof(GrowState), hipMemcpyDeviceToHost, e->stream) !=
+        hipSuccess)
+      return false;
+  return hipStreamSynchronize(e->stream) == hipSuccess;
+}
 void engine_grow_debug(usap_engine* e, long long out[4]) {
   out[0] = out[1] = out[2] = out[3] = 0;
-  for (DevSpM* m : {&e->M, &e->Mt}) {
-    GrowState g{};
-    if (hipMemcpy(&g, m->gst, sizeof(GrowState), hipMemcpyDeviceToHost) != hipSuccess) continue;
-    out[0] += (long long)g.grows;
-    out[1] += (long long)g.compactions;
-    out[2] += (long long)g.full;
-    out[3] += (long long)g.nospace_events;
+  GrowState g[2] = {};
+  if (!read_grow_states(e, g)) return;
+  for (int i = 0; i < 2; ++i) {
+    out[0] += (long long)g[i].grows;
+    out[1] += (long long)g[i].compactions;
+    out[2] += (long long)g[i].full;
+    out[3] += (long long)g[i].nospace_events;
   }
 }
 int64_t engine_grows(usap_engine* e) {
-  int64_t t = 0;
-  for (DevSpM* m : {&e->M, &e->Mt}) {
-    GrowState g{};
-    if (hipMemcpy(&g, m->gst, sizeof(GrowState), hipMemcpyDeviceToHost) == hipSuccess)
-      t += (int64_t)g.grows;
-  }
-  return t;
+  GrowState g[2] = {};
+  if (!read_grow_states(e, g)) return 0;
+  return (int64_t)(g[0].grows + g[1].grows);
 }
 void engine_set_error(usap_engine* e, const char* msg) { e->err = msg; }
 }  // namespace usap
@@ -2204,8 +2213,7 @@ int usap_destroy(usap_engine* e) {
     dfree(e, m->off);
     dfree(e, m->cap);
     dfree(e, m->nkeys);
-    dfree(e, m->key);
-    dfree(e, m->val);
+    dfree(e, m->kv);
     dfree(e, m->bs);
     dfree(e, m->gmark);
     dfree(e, m->gq);
@@ -2407,6 +2415,7 @@ int alloc_state(usap_engine* e) {
   hipStream_t st = e->stream;
   HIPCHK(hipMemsetAsync(e->status, 0, sizeof(Status), st));
   HIPCHK(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
+  for (DevSpM* m : {&e->M, &e->Mt}) HIPCHK(hipMemsetAsync(m->gst, 0, sizeof(GrowState), st));
   std::memset(e->h_sweep, 0, sizeof(Sweep));
   for (hipEvent_t& ev : e->ev) HIPCHK(hipEventCreate(&ev));
   return USAP_OK;
@@ -2526,12 +2535,11 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
   const int64_t used = std::min<int64_t>((int64_t)gs.used, e->M.P);
   std::vector<int64_t> off((size_t)B + 1);
   std::vector<int32_t> capv((size_t)B);
-  std::vector<int32_t> key((size_t)used), val((size_t)used);
+  std::vector<Slot> kv((size_t)used);
   HIPCHK(hipMemcpyAsync(off.data(), e->M.off, sizeof(int64_t) * (B + 1), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(capv.data(), e->M.cap, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
   if (used > 0) {
-    HIPCHK(hipMemcpyAsync(key.data(), e->M.key, sizeof(int32_t) * used, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(val.data(), e->M.val, sizeof(int32_t) * used, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(kv.data(), e->M.kv, sizeof(Slot) * used, hipMemcpyDeviceToHost, st));
   }
   HIPCHK(hipStreamSynchronize(st));
   std::vector<std::pair<int32_t, int32_t>> row;
@@ -2539,8 +2547,8 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
   for (int32_t a = 0; a < B; ++a) {
     row.clear();
     for (int64_t j = 0; j < capv[a]; ++j) {
-      const int32_t c = capv[a] == B ? (int32_t)j : key[off[a] + j];
-      const int32_t v = val[off[a] + j];
+      const int32_t c = capv[a] == B ? (int32_t)j : kv[off[a] + j].k;
+      const int32_t v = kv[off[a] + j].v;
       if (c >= 0 && v != 0) row.emplace_back(c, v);
     }
     std::sort(row.begin(), row.end());
@@ -2561,7 +2569,7 @@ int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
   if (int rc = check_engine(e)) return rc;
   hipStream_t st = e->stream;
   ev_rec(e, 0);
-  k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.val, e->M.gst, e->d_out, e->d_in, e->B, e->lam,
+  k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.kv, e->M.gst, e->d_out, e->d_in, e->B, e->lam,
                                            e->M.P, e->ent_part);
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
@@ -2771,8 +2779,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
     gs.off = ms[i]->off;
     gs.cap = ms[i]->cap;
     gs.nkeys = ms[i]->nkeys;
-    gs.key = ms[i]->key;
-    gs.val = ms[i]->val;
+    gs.kv = ms[i]->kv;
     gs.gmark = ms[i]->gmark;
     gs.gq = ms[i]->gq;
     gs.gpend = ms[i]->gpend;
