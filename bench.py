@@ -177,7 +177,7 @@ def batched_bench(n_graphs=256, workers=0, replay=0):
 
 
 def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interval_s=0.05,
-                 check=4, replay=0):
+                 check=4, replay=0, final_partition=True):
     """The served stream: n_graphs DCSBMs (N log-uniform in [1000, 200000])
     arriving by a Poisson process at each offered rate, each delivered as 10
     increments of 10% of its shuffled edges, interval_s apart; stage 1
@@ -195,7 +195,7 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
     for rate in rates:
         arr = sv.poisson_arrivals(len(graphs), rate, seed=5)
         labs, per, st = sv.serve(graphs, arr, params, stages=10, interval_s=interval_s,
-                                 workers=workers)
+                                 workers=workers, final_partition=final_partition)
         log(f"[bench] served at {rate} graphs/s: {st['graphs_per_s']:.2f} graphs/s, "
             f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, resident <= "
             f"{st['max_resident']}, wall {st['wall_s']:.1f}s")
@@ -209,7 +209,8 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
         mixes.append((labs, per))
     same = True
     for i in range(min(check, len(graphs))):
-        lab1, per1, _ = sv.serve([graphs[i]], [0.0], [params[i]], stages=10, workers=1)
+        lab1, per1, _ = sv.serve([graphs[i]], [0.0], [params[i]], stages=10, workers=1,
+                                 final_partition=final_partition)
         for labs, per in mixes:
             same &= bool(np.array_equal(labs[i], lab1[0]))
             same &= [r["S"] for r in per[i]] == [r["S"] for r in per1[0]]
@@ -227,7 +228,8 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
         "workload": (f"{len(graphs)} directed DCSBMs, N log-uniform in [1000, 200000], "
                      "round(N^0.4) planted blocks, p_in uniform in [0.55, 0.85], Poisson "
                      f"arrivals, 10 increments of 10% of the shuffled edges {interval_s}s apart; "
-                     "stage 1 partition (SCC init), stages 2-10 resume the chain"),
+                     "stage 1 partition (SCC init), stages 2-9 resume the chain, stage 10 "
+                     + ("partitions the whole graph" if final_partition else "resumes")),
     }
 
 

@@ -269,6 +269,8 @@ typedef struct usap_serve_opts {
   double interval_s;    /* time between a graph's increments */
   int32_t workers;      /* stages in flight at once (one host thread each) */
   int64_t hbm_budget;   /* device bytes for resident graphs by usap_footprint (0 = no limit) */
+  int32_t final_partition; /* 1: the last stage runs a full partition of the whole graph
+                              instead of resuming (the resumed chain keeps stage 1's B) */
 } usap_serve_opts;
 
 typedef struct usap_serve_stage {  /* one per (graph, stage), graph-major */
@@ -304,7 +306,8 @@ int64_t usap_footprint(int32_t N, int64_t E);
 /* Serves n graphs: graph g arrives at graphs[g].arrival_s and its stage k
  * (1..stages) becomes ready interval_s*(k-1) later.  Stage 1 = usap_create
  * + usap_partition; stage k > 1 = usap_set_graph + usap_resume (nodal
- * streams from k << 40), so the engine stays resident between stages.
+ * streams from k << 40), so the engine stays resident between stages; with
+ * final_partition the last stage is usap_set_graph + usap_partition.
  * Graphs are admitted in arrival order while their footprints fit
  * hbm_budget; up to `workers` ready stages of different graphs run at once,
  * oldest first.  Each graph's final labels go to labels + label_off[g];

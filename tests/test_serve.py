@@ -91,3 +91,19 @@ def test_serve_budget_bounds_resident_graphs(usap):
     labs2, per2, _ = sv.serve(gs, t, params, stages=2, workers=4)
     for a, b in zip(labs, labs2):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_serve_final_partition(usap):
+    """With final_partition the last stage is a full partition of the whole
+    graph on the resident engine: the same as partitioning it alone."""
+    gs, params = _small()
+    g, p = gs[2], params[2]
+    labs, per, st = sv.serve([g], [0.0], [p], stages=3, workers=1, final_partition=True)
+    assert st["failed"] == 0
+    full = usap.Graph.from_edges(g.src, g.dst, N=g.N)
+    eng = usap.Engine(full, **p)
+    lab, stt = eng.partition()
+    eng.close()
+    np.testing.assert_array_equal(lab, labs[0])
+    assert stt["S"] == per[0][2]["S"] and stt["B"] == per[0][2]["B"]

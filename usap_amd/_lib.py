@@ -154,6 +154,7 @@ class UsapServeOpts(C.Structure):
         ("interval_s", C.c_double),
         ("workers", C.c_int32),
         ("hbm_budget", C.c_int64),
+        ("final_partition", C.c_int32),
     ]
 
 

@@ -7,7 +7,9 @@
 // edges of its delivery order.  Stage 1 is a full partition()
 // (usap_partition); stage k > 1 replaces the engine's edges (usap_set_graph,
 // labels kept) and resumes the nodal chain from the previous stage's
-// partition (usap_resume).  A graph's engine -- labels, M, degrees -- stays
+// partition (usap_resume); optionally the last stage partitions the whole
+// graph afresh, since the resumed chain keeps the block count stage 1 chose
+// on a tenth of the edges.  A graph's engine -- labels, M, degrees -- stays
 // resident on the device from its first stage to its last.
 //
 // Continuous batching: `workers` host threads take whatever stage is ready
@@ -144,6 +146,9 @@ struct Server {
     int32_t* lab = labels + label_off[g];
     if (k == 1) {
       rc = usap_create(&ug, &G.params, &e);
+      if (rc == USAP_OK) rc = usap_partition(e, lab, &st);
+    } else if (k == opt.stages && opt.final_partition) {
+      rc = usap_set_graph(e, &ug);
       if (rc == USAP_OK) rc = usap_partition(e, lab, &st);
     } else {
       rc = usap_set_graph(e, &ug);
