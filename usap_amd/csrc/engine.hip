@@ -1000,6 +1000,24 @@ __global__ __launch_bounds__(256) void k_entropy(const Slot* __restrict__ kv,
   block_store128(acc, part + 2 * blockIdx.x);
 }
 
+// Carry-out order (graph_parallel.hpp:1230) on the device: blocks sorted by
+// (dS, block id), dS an int128 ordered as (signed hi, unsigned lo), by two
+// stable radix passes -- lo first, then hi with its sign bit flipped.
+__global__ void k_sort_init(int32_t B, const unsigned long* __restrict__ lo,
+                            unsigned long long* key, int32_t* idx) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < B) {
+    key[i] = lo[i];
+    idx[i] = i;
+  }
+}
+
+__global__ void k_sort_hi(int32_t B, const long* __restrict__ hi, const int32_t* __restrict__ perm,
+                          unsigned long long* key) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < B) key[i] = (unsigned long long)hi[perm[i]] ^ (1ull << 63);
+}
+
 // ---------------------------------------------------------------------------
 // Block merge, graph_parallel.hpp:761-988 (+ :349-368)
 // ---------------------------------------------------------------------------
@@ -1832,6 +1850,10 @@ struct usap_engine {
   unsigned long* best_lo = nullptr;
   int32_t merge_B = -1;  // B of the last merge_propose (-1: none)
   int32_t* map = nullptr;
+  unsigned long long *sort_key[2] = {nullptr, nullptr};  // carry-out sort buffers [N]
+  int32_t* sort_idx[2] = {nullptr, nullptr};
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
   // nodal
   int32_t* prop_s = nullptr;
   long* prop_hi = nullptr;
@@ -2228,6 +2250,11 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->best_hi);
   dfree(e, e->best_lo);
   dfree(e, e->map);
+  for (int i = 0; i < 2; ++i) {
+    dfree(e, e->sort_key[i]);
+    dfree(e, e->sort_idx[i]);
+  }
+  dfree(e, e->sort_tmp);
   dfree(e, e->prop_s);
   dfree(e, e->prop_hi);
   dfree(e, e->prop_lo);
@@ -2646,22 +2673,37 @@ int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new) {
   const int32_t B = e->B;
   if (e->merge_B != B) return set_err(e, USAP_ESTATE, "merge_apply without merge_propose");
   if (B_to_merge < 0 || B_to_merge >= B) return set_err(e, USAP_EINVAL, "B_to_merge out of range");
-  std::vector<int32_t> best((size_t)B), bsize((size_t)B);
-  std::vector<long> h((size_t)B);
-  std::vector<unsigned long> l((size_t)B);
+  std::vector<int32_t> best((size_t)B), bsize((size_t)B), order((size_t)B);
   hipStream_t st = e->stream;
+  // order by (dS, block id) on the device: the reference sorts by dS with an
+  // unstable sort (graph_parallel.hpp:1230)
+  if (!e->sort_idx[0]) {
+    for (int i = 0; i < 2; ++i) {
+      if (int rc = dalloc(e, &e->sort_key[i], e->N)) return rc;
+      if (int rc = dalloc(e, &e->sort_idx[i], e->N)) return rc;
+    }
+  }
+  size_t bytes = 0;
+  HIPCHK(rocprim::radix_sort_pairs(nullptr, bytes, e->sort_key[0], e->sort_key[1], e->sort_idx[0],
+                                   e->sort_idx[1], (size_t)B, 0, 64, st));
+  if (bytes > e->sort_tmp_bytes) {
+    dfree(e, e->sort_tmp);
+    HIPCHK(hipMallocAsync(&e->sort_tmp, bytes, st));
+    e->sort_tmp_bytes = bytes;
+  }
+  k_sort_init<<<blocks_for(B, 256), 256, 0, st>>>(B, e->best_lo, e->sort_key[0], e->sort_idx[0]);
+  bytes = e->sort_tmp_bytes;
+  HIPCHK(rocprim::radix_sort_pairs(e->sort_tmp, bytes, e->sort_key[0], e->sort_key[1],
+                                   e->sort_idx[0], e->sort_idx[1], (size_t)B, 0, 64, st));
+  k_sort_hi<<<blocks_for(B, 256), 256, 0, st>>>(B, e->best_hi, e->sort_idx[1], e->sort_key[0]);
+  bytes = e->sort_tmp_bytes;
+  HIPCHK(rocprim::radix_sort_pairs(e->sort_tmp, bytes, e->sort_key[0], e->sort_key[1],
+                                   e->sort_idx[1], e->sort_idx[0], (size_t)B, 0, 64, st));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(order.data(), e->sort_idx[0], sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(best.data(), e->best_s, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(h.data(), e->best_hi, sizeof(long) * B, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(l.data(), e->best_lo, sizeof(long) * B, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(bsize.data(), e->bsize, sizeof(int32_t) * B, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  // order by (dS, block id): the reference sorts by dS with an unstable sort
-  std::vector<int32_t> order((size_t)B);
-  std::iota(order.begin(), order.end(), 0);
-  std::sort(order.begin(), order.end(), [&](int32_t i, int32_t j) {
-    const i128 a = mk128(h[i], l[i]), b = mk128(h[j], l[j]);
-    return a < b || (a == b && i < j);
-  });
   // greedy merges: block_map of graph_parallel.hpp:1236-1257 as a union-find
   // whose root is the representative (the merge target's current block)
   std::vector<int32_t> parent((size_t)B);
