@@ -26,10 +26,10 @@
 //   k_nodal   : one nodal proposal per node against the frozen M
 //        (graph_parallel.hpp:990-1222), one wavefront per node
 //   k_apply   : accepted moves -> M, degrees (int32 atomics) and labels;
-//        equals the reference's full rebuild (graph_parallel.hpp:438-443); its
-//        last block runs the sweep's update step and convergence test
+//        equals the reference's full rebuild (graph_parallel.hpp:438-443)
+//   k_grow    : row growth / compaction and overflow replay between batches;
+//        its block 0 runs the sweep's update step and convergence test
 //        (graph_parallel.hpp:433-458) so a whole sweep runs without the host
-//   k_grow    : row growth / compaction and overflow replay between batches
 //   k_entropy : data_S over all non-zeros (graph_parallel.hpp:1281-1319)
 //
 // The nodal and apply kernels are latency bound (a few hundred bytes per node
@@ -88,7 +88,7 @@ struct Status {
 };
 
 // Device state of one nodal sweep (the reference's `update` task,
-// graph_parallel.hpp:433-458, run by the last block of each k_apply).
+// graph_parallel.hpp:433-458, run by block 0 of each batch's k_grow).
 struct Sweep {
   unsigned long long cum_lo, cum_hi;  // exact cumulative dS of the sweep (int128)
   unsigned long long stream0;         // random stream of the sweep's first batch
@@ -103,7 +103,7 @@ struct Sweep {
   int32_t stop;     // 0 run, 1 converged, 2 M needs a rebuild, 3 inconsistent M
   int32_t done;     // batches executed
   int32_t flags;    // OR of the batches' flags
-  uint32_t arrive;  // blocks of the current k_apply that finished
+  int32_t applied;  // k_apply ran in the current batch (the update step is due)
 };
 
 struct GraphView {
@@ -146,7 +146,6 @@ struct GrowState {
   int32_t n;                  // rows queued by the last k_apply
   int32_t nospace;            // the pool ran out: the host rebuilds M
   int32_t ovf_n;              // entries in the overflow buffer
-  uint32_t arrive;            // blocks of the current k_grow that finished
   unsigned long long used;    // pool slots in use
   unsigned long long grows;   // rows grown or compacted since the engine was created
   unsigned long long full;    // insertions that found no free slot
@@ -752,206 +751,6 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   for (int i = lane; i < kNBK; i += kWave) m.bs[a * kNBK + i] = bk[i];
 }
 
-// Rows queued by k_apply (key count past 1/4 of the slab, or an increment
-// parked), handled between two batches, zero counts dropped: a row whose live
-// keys and parked increments fit an eighth of its slab (and at most kCompact
-// slots) is compacted in place through LDS; any other
-// row moves to a slab >= 8*live+16 (direct once that reaches B/8) at the free
-// tail of the pool.  Bucket sums are unchanged.  One launch serves M and M^T.
-constexpr int kCompact = 8192;
-constexpr int32_t kOvfCap = 1 << 22;  // parked increments per orientation
-
-struct GrowSide {
-  int64_t* off;
-  int32_t* cap;
-  int32_t* nkeys;
-  Slot* kv;
-  int32_t* gmark;
-  const int32_t* gq;
-  int32_t* gpend;
-  const int32_t* ovf;
-  int32_t ovf_cap;
-  GrowState* gst;
-  int64_t P;
-};
-
-struct GrowArgs {
-  GrowSide side[2];
-  int32_t B;
-  Sweep* sweep;
-};
-
-__global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
-  __shared__ int32_t s_k[kCompact];
-  __shared__ int32_t s_v[kCompact];
-  __shared__ long long s_base;
-  __shared__ int32_t s_cap, s_live;
-  const int32_t B = ga.B;
-  const int32_t n0 = __hip_atomic_load(&ga.side[0].gst->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int32_t n1 = __hip_atomic_load(&ga.side[1].gst->n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int32_t qi = blockIdx.x; qi < n0 + n1; qi += gridDim.x) {
-    const GrowSide& g = qi < n0 ? ga.side[0] : ga.side[1];
-    const int32_t a = g.gq[qi < n0 ? qi : qi - n0];
-    const int64_t o = g.off[a];
-    const int32_t c = g.cap[a];
-    const bool dold = c == B;
-    if (threadIdx.x == 0) s_live = 0;
-    __syncthreads();
-    for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-      const int32_t k = dold ? j : g.kv[o + j].k;
-      if (k >= 0 && g.kv[o + j].v != 0) atomicAdd(&s_live, 1);
-    }
-    __syncthreads();
-    const int32_t live = s_live;
-    const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
-    if (!dold && 8 * (live + pend) <= c && c <= kCompact) {
-      // in place: gather the live entries, clear, reinsert
-      __syncthreads();
-      if (threadIdx.x == 0) s_live = 0;
-      __syncthreads();
-      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        const Slot x = g.kv[o + j];
-        const int32_t k = x.k, v = x.v;
-        if (k >= 0 && v != 0) {
-          const int32_t i = atomicAdd(&s_live, 1);
-          s_k[i] = k;
-          s_v[i] = v;
-        }
-      }
-      __syncthreads();
-      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        g.kv[o + j] = Slot{-1, 0};
-      }
-      __threadfence_block();
-      __syncthreads();
-      const uint32_t mask = (uint32_t)c - 1;
-      for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
-        uint32_t h = hash32(s_k[i]) & mask;
-        while (atomicCAS(&g.kv[o + h].k, -1, s_k[i]) != -1) h = (h + 1) & mask;
-        g.kv[o + h].v = s_v[i];
-      }
-      __threadfence_block();
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        g.nkeys[a] = live;
-        g.gmark[a] = 0;
-        g.gpend[a] = 0;
-        g.gst->grows += 1;
-        g.gst->compactions += 1;
-      }
-      __syncthreads();
-      continue;
-    }
-    if (threadIdx.x == 0) {
-      int64_t nc = 32;
-      while (nc < 8 * ((int64_t)live + pend) + 16) nc <<= 1;
-      if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
-      if (8 * nc >= B) nc = B;
-      const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
-      if (dold || (int64_t)base + nc > g.P) {
-        atomicAdd(&g.gst->used, (unsigned long long)(-nc));  // give the slots back
-        s_cap = 0;
-        if (!dold) {
-          g.gst->nospace = 1;
-          g.gst->nospace_events += 1;
-          atomicOr(&ga.sweep->flags, kFlagFull);
-          if (ga.sweep->stop == 0) ga.sweep->stop = 2;
-        }
-      } else {
-        s_cap = (int32_t)nc;
-        s_base = (long long)base;
-      }
-    }
-    __syncthreads();
-    const int32_t nc = s_cap;
-    if (nc > 0) {
-      const int64_t nb = s_base;
-      for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) {
-        g.kv[nb + i] = Slot{-1, 0};
-      }
-      __threadfence_block();
-      __syncthreads();
-      const bool dnew = nc == B;
-      const uint32_t mask = (uint32_t)nc - 1;
-      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        const Slot x = g.kv[o + j];
-        const int32_t k = x.k, v = x.v;
-        if (k < 0 || v == 0) continue;
-        g.kv[o + j].v = 0;  // the old slab stays in the pool as zeros until a rebuild
-        if (dnew) {
-          g.kv[nb + k].v = v;
-          continue;
-        }
-        uint32_t h = hash32(k) & mask;
-        while (atomicCAS(&g.kv[nb + h].k, -1, k) != -1) h = (h + 1) & mask;
-        g.kv[nb + h].v = v;
-      }
-      __threadfence_block();
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        g.off[a] = nb;
-        g.cap[a] = nc;
-        g.nkeys[a] = live;
-        g.gst->grows += 1;
-      }
-    }
-    if (threadIdx.x == 0) {
-      g.gmark[a] = 0;
-      g.gpend[a] = 0;
-    }
-    __syncthreads();
-  }
-  // the last block replays the parked increments into the grown rows
-  __shared__ int s_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(&ga.side[0].gst->arrive, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int side = 0; side < 2; ++side) {
-    const GrowSide& g = ga.side[side];
-    const int32_t no = min(__hip_atomic_load(&g.gst->ovf_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                           g.ovf_cap);
-    for (int32_t i = threadIdx.x; i < no; i += blockDim.x) {
-      const int32_t a = g.ovf[i], b = g.ovf[g.ovf_cap + i], w = g.ovf[2 * g.ovf_cap + i];
-      const int64_t o = g.off[a];
-      const int32_t c = g.cap[a];
-      if (c == B) {
-        atomicAdd(&g.kv[o + b].v, w);
-        continue;
-      }
-      const uint32_t mask = (uint32_t)c - 1;
-      uint32_t h = hash32(b) & mask;
-      bool done = false;
-      for (int32_t probe = 0; probe < c && !done; ++probe) {
-        int32_t k = __hip_atomic_load(&g.kv[o + h].k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k < 0) {
-          k = atomicCAS(&g.kv[o + h].k, -1, b);
-          if (k == -1) {
-            atomicAdd(&g.nkeys[a], 1);
-            k = b;
-          }
-        }
-        if (k == b) {
-          atomicAdd(&g.kv[o + h].v, w);
-          done = true;
-        }
-        h = (h + 1) & mask;
-      }
-      if (!done) {  // still no room (the pool ran out): rebuild M
-        atomicOr(&ga.sweep->flags, kFlagFull);
-        if (ga.sweep->stop == 0) ga.sweep->stop = 2;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) g.gst->ovf_n = 0;
-  }
-  if (threadIdx.x == 0) ga.side[0].gst->arrive = 0;
-}
-
 __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
   const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < N) labels[x] = map[labels[x]];
@@ -1155,7 +954,7 @@ constexpr int kEdgeRegs = 4;     // list entries per lane per pass (256 per wave
 constexpr int kScrNodal = 1024;  // k_nodal's LDS scratch per wavefront (one key bucket)
 
 // One workgroup's share of a batch, written by k_nodal / k_apply and summed
-// by the last block of k_apply (no same-address atomics per node).
+// by block 0 of k_grow (no same-address atomics per node).
 struct BlockPart {
   long sum_hi;  // sum of the accepted finite dS of the block's nodes (int128)
   unsigned long sum_lo;
@@ -1500,6 +1299,8 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
     a.M.gst->n = 0;
     a.Mt.gst->n = 0;
+    a.M.gst->ovf_n = 0;
+    a.Mt.gst->ovf_n = 0;
   }
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
@@ -1573,13 +1374,16 @@ __device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_
   sw->S += dS_sum;
   sw->done += 1;
   sw->moves += (unsigned long long)moves;
-  sw->flags |= flags;
+  // k_grow's other blocks may raise flags / ask for a rebuild meanwhile:
+  // convergence and errors override a rebuild request, as the step that
+  // decides comes after the batch
+  atomicOr(&sw->flags, flags);
   if (flags & (kFlagMissing | kFlagNegative))
-    sw->stop = 3;
+    atomicExch(&sw->stop, 3);
   else if (sw->check && -sw->mean < sw->thr * sw->S)
-    sw->stop = 1;
+    atomicExch(&sw->stop, 1);
   else if (flags & kFlagFull)
-    sw->stop = 2;
+    atomicCAS(&sw->stop, 0, 2);
   st->flags = 0;
 }
 
@@ -1695,9 +1499,10 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
   return 52 + 60 * kc + 32 + 32 + 4;
 }
 
-// The batch's sums over all workgroups' shares, by the last block of k_apply,
-// then the sweep step.
-__device__ __forceinline__ void batch_tail(const ApplyArgs& a) {
+// The batch's sums over all workgroups' shares (nparts of them), by one
+// block, then the sweep step.
+__device__ __forceinline__ void batch_tail(const BlockPart* part, uint32_t nparts, Status* st,
+                                           Sweep* sw) {
   __shared__ unsigned long s_lo[kNodalWaves];
   __shared__ long s_hi[kNodalWaves];
   __shared__ unsigned long long s_nb[kNodalWaves], s_ab[kNodalWaves];
@@ -1705,15 +1510,13 @@ __device__ __forceinline__ void batch_tail(const ApplyArgs& a) {
   const int w = threadIdx.x / kWave;
   i128 sum = 0;
   long nb = 0, ab = 0, mv = 0, ni = 0;
-  for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-    BlockPart* P = &a.part[i];
-    const unsigned long long lo = __hip_atomic_load((unsigned long long*)&P->sum_lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long hi = __hip_atomic_load((unsigned long long*)&P->sum_hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sum += (i128)(((u128)hi << 64) | (u128)lo);
-    nb += (long)__hip_atomic_load(&P->nodal_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ab += (long)__hip_atomic_load(&P->apply_bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    mv += __hip_atomic_load(&P->moves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ni += __hip_atomic_load(&P->ninf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+    const BlockPart P = part[i];
+    sum += (i128)(((u128)(unsigned long)P.sum_hi << 64) | (u128)P.sum_lo);
+    nb += (long)P.nodal_bytes;
+    ab += (long)P.apply_bytes;
+    mv += P.moves;
+    ni += P.ninf;
   }
   sum = wave_sum128(sum);
   nb = wave_sum(nb);
@@ -1733,50 +1536,59 @@ __device__ __forceinline__ void batch_tail(const ApplyArgs& a) {
     u128 tot = 0;
     unsigned long long tnb = 0, tab = 0;
     long tmv = 0, tni = 0;
-    for (int i = 0; i < kNodalWaves; ++i) {
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) {
       tot += ((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i];
       tnb += s_nb[i];
       tab += s_ab[i];
       tmv += s_mv[i];
       tni += s_ni[i];
     }
-    sweep_tail(a.sweep, a.status, tot, (int32_t)tni, (int32_t)tmv, tnb, tab);
-    a.sweep->arrive = 0;
-    __threadfence();
+    sweep_tail(sw, st, tot, (int32_t)tni, (int32_t)tmv, tnb, tab);
   }
+  __syncthreads();
 }
 
 __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
-  __shared__ unsigned long long s_ab[kNodalWaves];
-  __shared__ int s_last;
+  __shared__ unsigned long long s_ab[kNodalWaves], s_lo[kNodalWaves];
+  __shared__ long s_hi[kNodalWaves];
+  __shared__ int32_t s_mv[kNodalWaves], s_ni[kNodalWaves];
   if (a.sweep->stop) return;  // the same for every block of this launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.sweep->applied = 1;
   const int w = threadIdx.x / kWave;
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
   long bytes = 0;
+  int32_t mv = 0, ni = 0;
+  u128 dsum = 0;
   if (n64 < a.end && a.prop_acc[item]) {
     const int32_t n = (int32_t)n64;
+    mv = 1;
+    if (a.prop_inf[item])
+      ni = 1;
+    else
+      dsum = ((u128)(unsigned long)a.prop_hi[item] << 64) | (u128)a.prop_lo[item];
     const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
     bytes = kc > kTabMaxDeg ? apply_one<true>(a, n, a.prop_r[item], a.prop_s[item], s_tab[w])
                             : apply_one<false>(a, n, a.prop_r[item], a.prop_s[item], s_tab[w]);
   }
-  if (lane_id() == 0) s_ab[w] = (unsigned long long)bytes;
+  if (lane_id() == 0) {
+    s_ab[w] = (unsigned long long)bytes;
+    s_lo[w] = (unsigned long long)dsum;
+    s_hi[w] = (long)(dsum >> 64);
+    s_mv[w] = mv;
+    s_ni[w] = ni;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {  // this block's share of the batch sums
+  if (threadIdx.x == 0) {  // this block's share of the batch sums, for k_grow's block 0
     u128 sum = 0;
     unsigned long long ab = 0;
     int32_t moves = 0, ninf = 0;
     for (int i = 0; i < kNodalWaves; ++i) {
-      const int64_t it = (int64_t)blockIdx.x * kNodalWaves + i;
       ab += s_ab[i];
-      if (a.beg + it < a.end && a.prop_acc[it]) {
-        ++moves;
-        if (a.prop_inf[it])
-          ++ninf;
-        else
-          sum += ((u128)(unsigned long)a.prop_hi[it] << 64) | (u128)a.prop_lo[it];
-      }
+      sum += ((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i];
+      moves += s_mv[i];
+      ninf += s_ni[i];
     }
     BlockPart& P = a.part[blockIdx.x];
     P.sum_lo = (unsigned long)sum;
@@ -1784,13 +1596,209 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
     P.apply_bytes = ab;
     P.moves = moves;
     P.ninf = ninf;
-    __threadfence();
-    s_last = atomicAdd(&a.sweep->arrive, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
-    batch_tail(a);
+}
+
+// Rows queued by k_apply (key count past 1/4 of the slab, or an increment
+// parked), handled between two batches, zero counts dropped: a row whose live
+// keys and parked increments fit an eighth of its slab (and at most kCompact
+// slots) is compacted in place through LDS; any other
+// row moves to a slab >= 8*live+16 (direct once that reaches B/8) at the free
+// tail of the pool, and the row's parked increments are replayed into it.
+// Bucket sums are unchanged.  One launch serves M and M^T; its block 0 first
+// runs the batch's update step.
+constexpr int kCompact = 8192;
+constexpr int32_t kOvfCap = 1 << 22;  // parked increments per orientation
+
+struct GrowSide {
+  int64_t* off;
+  int32_t* cap;
+  int32_t* nkeys;
+  Slot* kv;
+  int32_t* gmark;
+  const int32_t* gq;
+  int32_t* gpend;
+  const int32_t* ovf;
+  int32_t ovf_cap;
+  GrowState* gst;
+  int64_t P;
+};
+
+struct GrowArgs {
+  GrowSide side[2];
+  int32_t B;
+  Sweep* sweep;
+  const BlockPart* part;  // k_apply's per-workgroup shares of the batch
+  uint32_t nparts;
+  Status* status;
+};
+
+// Replays the parked increments of row a of one orientation into its slab
+// (o, c) (the only block touching row a in this launch); an increment that
+// still finds no slot asks the host for a rebuild.
+__device__ void replay_row(const GrowSide& g, int32_t a, int64_t o, int32_t c, int32_t B,
+                           Sweep* sw) {
+  const int32_t no = min(g.gst->ovf_n, g.ovf_cap);
+  for (int32_t i = threadIdx.x; i < no; i += blockDim.x) {
+    if (g.ovf[i] != a) continue;
+    const int32_t b = g.ovf[g.ovf_cap + i], w = g.ovf[2 * g.ovf_cap + i];
+    if (c == B) {
+      atomicAdd(&g.kv[o + b].v, w);
+      continue;
+    }
+    const uint32_t mask = (uint32_t)c - 1;
+    uint32_t h = hash32(b) & mask;
+    bool done = false;
+    for (int32_t probe = 0; probe < c && !done; ++probe) {
+      int32_t k = __hip_atomic_load(&g.kv[o + h].k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k < 0) {
+        k = atomicCAS(&g.kv[o + h].k, -1, b);
+        if (k == -1) {
+          atomicAdd(&g.nkeys[a], 1);
+          k = b;
+        }
+      }
+      if (k == b) {
+        atomicAdd(&g.kv[o + h].v, w);
+        done = true;
+      }
+      h = (h + 1) & mask;
+    }
+    if (!done) {  // no room (the pool ran out): rebuild M
+      atomicOr(&sw->flags, kFlagFull);
+      atomicCAS(&sw->stop, 0, 2);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
+  __shared__ int32_t s_k[kCompact];
+  __shared__ int32_t s_v[kCompact];
+  __shared__ long long s_base;
+  __shared__ int32_t s_cap, s_live;
+  const int32_t B = ga.B;
+  // block 0: the batch's sums and the sweep step, if k_apply ran this batch
+  // (its per-workgroup shares were complete when this kernel started)
+  if (blockIdx.x == 0) {
+    const bool applied = ga.sweep->applied != 0;
+    __syncthreads();
+    if (applied) {
+      batch_tail(ga.part, ga.nparts, ga.status, ga.sweep);
+      if (threadIdx.x == 0) ga.sweep->applied = 0;
+    }
+  }
+  const int32_t n0 = ga.side[0].gst->n;
+  const int32_t n1 = ga.side[1].gst->n;
+  for (int32_t qi = blockIdx.x; qi < n0 + n1; qi += gridDim.x) {
+    const GrowSide& g = qi < n0 ? ga.side[0] : ga.side[1];
+    const int32_t a = g.gq[qi < n0 ? qi : qi - n0];
+    const int64_t o = g.off[a];
+    const int32_t c = g.cap[a];
+    const bool dold = c == B;
+    if (threadIdx.x == 0) s_live = 0;
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+      const int32_t k = dold ? j : g.kv[o + j].k;
+      if (k >= 0 && g.kv[o + j].v != 0) atomicAdd(&s_live, 1);
+    }
+    __syncthreads();
+    const int32_t live = s_live;
+    const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
+    int64_t o2 = o;                   // the row's slab after this step
+    int32_t c2 = c;
+    if (!dold && 8 * (live + pend) <= c && c <= kCompact) {
+      // in place: gather the live entries, clear, reinsert
+      __syncthreads();
+      if (threadIdx.x == 0) s_live = 0;
+      __syncthreads();
+      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+        const Slot x = g.kv[o + j];
+        const int32_t k = x.k, v = x.v;
+        if (k >= 0 && v != 0) {
+          const int32_t i = atomicAdd(&s_live, 1);
+          s_k[i] = k;
+          s_v[i] = v;
+        }
+      }
+      __syncthreads();
+      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) g.kv[o + j] = Slot{-1, 0};
+      __threadfence_block();
+      __syncthreads();
+      const uint32_t mask = (uint32_t)c - 1;
+      for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
+        uint32_t h = hash32(s_k[i]) & mask;
+        while (atomicCAS(&g.kv[o + h].k, -1, s_k[i]) != -1) h = (h + 1) & mask;
+        g.kv[o + h].v = s_v[i];
+      }
+      __threadfence_block();
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        g.nkeys[a] = live;
+        g.gst->grows += 1;
+        g.gst->compactions += 1;
+      }
+    } else {
+      if (threadIdx.x == 0) {
+        int64_t nc = 32;
+        while (nc < 8 * ((int64_t)live + pend) + 16) nc <<= 1;
+        if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
+        if (8 * nc >= B) nc = B;
+        const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
+        if (dold || (int64_t)base + nc > g.P) {
+          atomicAdd(&g.gst->used, (unsigned long long)(-nc));  // give the slots back
+          s_cap = 0;
+          if (!dold) {
+            g.gst->nospace = 1;
+            g.gst->nospace_events += 1;
+            atomicOr(&ga.sweep->flags, kFlagFull);
+            atomicCAS(&ga.sweep->stop, 0, 2);
+          }
+        } else {
+          s_cap = (int32_t)nc;
+          s_base = (long long)base;
+        }
+      }
+      __syncthreads();
+      const int32_t nc = s_cap;
+      if (nc > 0) {
+        const int64_t nb = s_base;
+        for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) g.kv[nb + i] = Slot{-1, 0};
+        __threadfence_block();
+        __syncthreads();
+        const bool dnew = nc == B;
+        const uint32_t mask = (uint32_t)nc - 1;
+        for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+          const Slot x = g.kv[o + j];
+          const int32_t k = x.k, v = x.v;
+          if (k < 0 || v == 0) continue;
+          g.kv[o + j].v = 0;  // the old slab stays in the pool as zeros until a rebuild
+          if (dnew) {
+            g.kv[nb + k].v = v;
+            continue;
+          }
+          uint32_t h = hash32(k) & mask;
+          while (atomicCAS(&g.kv[nb + h].k, -1, k) != -1) h = (h + 1) & mask;
+          g.kv[nb + h].v = v;
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          g.off[a] = nb;
+          g.cap[a] = nc;
+          g.nkeys[a] = live;
+          g.gst->grows += 1;
+        }
+        o2 = nb;
+        c2 = nc;
+      }
+    }
+    if (pend > 0) replay_row(g, a, o2, c2, B, ga.sweep);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      g.gmark[a] = 0;
+      g.gpend[a] = 0;
+    }
+    __syncthreads();
   }
 }
 
@@ -2090,7 +2098,6 @@ int rebuild_M(usap_engine* e) {
     gs0.n = 0;
     gs0.nospace = 0;
     gs0.ovf_n = 0;
-    gs0.arrive = 0;
     gs0.used = (unsigned long long)m.total;
     HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -2832,6 +2839,9 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   }
   ga.B = B;
   ga.sweep = e->sweep;
+  ga.part = e->part;
+  ga.nparts = grid;
+  ga.status = e->status;
   if (e->prof) {
     // start/stop events attached to each launch time the kernel itself
     while (e->bev.size() < 6 * (ev_slot + 1)) {
