@@ -326,9 +326,9 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
                         int32_t max_workers, int32_t sample, int32_t* workers, int32_t* tried,
                         double* rates);
 
-/* Kernel profile: time on the engine's stream (nodal, apply and grow: each
- * kernel's own execution, from events attached to its launch; the others:
- * events around their launches), launches, and the
+/* Kernel profile: time on the engine's stream (nodal, apply and grow: from
+ * the launch's first workgroup start to its last workgroup end, stamped in
+ * the kernel; the others: events around their launches), launches, and the
  * algorithmic bytes the kernel moved (reads + writes the algorithm needs,
  * each counted once; 0 where not tallied). */
 #define USAP_K_NODAL 0   /* nodal proposals */

@@ -751,6 +751,48 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   for (int i = lane; i < kNBK; i += kWave) m.bs[a * kNBK + i] = bk[i];
 }
 
+// Workgroup stamps of profiled batch launches (100 MHz wall clock; 0 = the
+// workgroup did not run).  Each workgroup of a profiled k_nodal, k_apply or
+// k_grow writes its start at t[b] and its end at t[stride + b] with plain
+// stores: one atomic per workgroup on a shared word costs more than the
+// kernel's own tail.  Per batch slot: k_nodal's starts and ends (G each),
+// k_apply's (G each), k_grow's (kGrowBlocks each).
+__global__ void k_stamp_init(unsigned long long* t, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    t[i] = 0ull;
+}
+
+// Per batch slot and kernel (block slot*3 + kernel): the first workgroup start
+// and the last workgroup end, to out[6*slot + 2*kernel + {0, 1}].
+__global__ __launch_bounds__(256) void k_stamp_reduce(const unsigned long long* t, int32_t G,
+                                                     int32_t grow_blocks, unsigned long long* out) {
+  __shared__ unsigned long long s_lo[256], s_hi[256];
+  const int32_t slot = blockIdx.x / 3, k = blockIdx.x % 3;
+  const int64_t base = (int64_t)slot * (4 * (int64_t)G + 2 * grow_blocks) + (int64_t)k * 2 * G;
+  const int32_t n = k < 2 ? G : grow_blocks;
+  unsigned long long lo = ~0ull, hi = 0ull;
+  for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const unsigned long long a = t[base + i], b = t[base + n + i];
+    if (a != 0ull) lo = min(lo, a);
+    hi = max(hi, b);
+  }
+  s_lo[threadIdx.x] = lo;
+  s_hi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_lo[threadIdx.x] = min(s_lo[threadIdx.x], s_lo[threadIdx.x + h]);
+      s_hi[threadIdx.x] = max(s_hi[threadIdx.x], s_hi[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[6 * (int64_t)slot + 2 * k] = s_lo[0];
+    out[6 * (int64_t)slot + 2 * k + 1] = s_hi[0];
+  }
+}
+
 __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
   const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < N) labels[x] = map[labels[x]];
@@ -987,6 +1029,10 @@ struct NodalArgs {
   BlockPart* part;
   Status* status;
   const Sweep* sweep;
+  // kernel timers on (profiling): [0] earliest workgroup start, [1] latest
+  // workgroup end of this launch, 100 MHz clock; else null
+  unsigned long long* tstamp;  // workgroup stamps (start [b], end [tstride + b]) or null
+  int32_t tstride;
 };
 
 __device__ __forceinline__ void tab_add(int32_t* tk, int32_t* tv, uint32_t mask, int32_t b,
@@ -1296,6 +1342,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
   __shared__ __align__(8) int32_t s_scr[kNodalWaves][kScrNodal];
   const int w = threadIdx.x / kWave;
+  if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
   if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
     a.M.gst->n = 0;
     a.Mt.gst->n = 0;
@@ -1319,6 +1366,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     unsigned long long t = 0;
     for (int i = 0; i < kNodalWaves; ++i) t += s_bytes[i];
     a.part[blockIdx.x].nodal_bytes = t;
+    if (a.tstamp) a.tstamp[a.tstride + blockIdx.x] = wall_clock64();
   }
 }
 
@@ -1346,6 +1394,8 @@ struct ApplyArgs {
   BlockPart* part;
   Status* status;
   Sweep* sweep;
+  unsigned long long* tstamp;  // as NodalArgs::tstamp, for this kernel
+  int32_t tstride;
 };
 
 __device__ double fx_to_double_dev(i128 v) {
@@ -1554,6 +1604,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   __shared__ long s_hi[kNodalWaves];
   __shared__ int32_t s_mv[kNodalWaves], s_ni[kNodalWaves];
   if (a.sweep->stop) return;  // the same for every block of this launch
+  if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
   if (blockIdx.x == 0 && threadIdx.x == 0) a.sweep->applied = 1;
   const int w = threadIdx.x / kWave;
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
@@ -1596,6 +1647,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
     P.apply_bytes = ab;
     P.moves = moves;
     P.ninf = ninf;
+    if (a.tstamp) a.tstamp[a.tstride + blockIdx.x] = wall_clock64();
   }
 }
 
@@ -1608,6 +1660,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
 // Bucket sums are unchanged.  One launch serves M and M^T; its block 0 first
 // runs the batch's update step.
 constexpr int kCompact = 8192;
+constexpr int kGrowBlocks = 128;  // k_grow's grid
 constexpr int32_t kOvfCap = 1 << 22;  // parked increments per orientation
 
 struct GrowSide {
@@ -1631,6 +1684,8 @@ struct GrowArgs {
   const BlockPart* part;  // k_apply's per-workgroup shares of the batch
   uint32_t nparts;
   Status* status;
+  unsigned long long* tstamp;  // as NodalArgs::tstamp, for this kernel
+  int32_t tstride;
 };
 
 // Replays the parked increments of row a of one orientation into its slab
@@ -1677,6 +1732,7 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
   __shared__ long long s_base;
   __shared__ int32_t s_cap, s_live;
   const int32_t B = ga.B;
+  if (ga.tstamp && threadIdx.x == 0) ga.tstamp[blockIdx.x] = wall_clock64();
   // block 0: the batch's sums and the sweep step, if k_apply ran this batch
   // (its per-workgroup shares were complete when this kernel started)
   if (blockIdx.x == 0) {
@@ -1800,6 +1856,10 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     }
     __syncthreads();
   }
+  if (ga.tstamp) {
+    __syncthreads();
+    if (threadIdx.x == 0) ga.tstamp[ga.tstride + blockIdx.x] = wall_clock64();
+  }
 }
 
 }  // namespace
@@ -1874,7 +1934,9 @@ struct usap_engine {
   Sweep* sweep = nullptr;     // device sweep state
   Sweep* h_sweep = nullptr;   // pinned host copy
   int32_t* prop_r = nullptr;  // old label of each batch node
-  std::vector<hipEvent_t> bev;  // per-batch kernel events (profiling)
+  unsigned long long* tstamp = nullptr;  // profiled batch launches: workgroup stamps
+  size_t tstamp_n = 0;                   // its length
+  int32_t tstamp_G = 0;                  // k_nodal / k_apply workgroups per slot this round
   // captured sweep (usap_params.graphs): replayed while its key matches
   hipGraphExec_t gexec = nullptr;
   int32_t g_B = -1, g_batch = -1, g_nb = -1;
@@ -2272,8 +2334,8 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->sweep);
   dfree(e, e->prop_r);
   dfree(e, e->build_flags);
-  for (hipEvent_t ev : e->bev) (void)hipEventDestroy(ev);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  dfree(e, e->tstamp);
   pin_put(e->h_sweep, sizeof(Sweep));
   dfree(e, e->big_off);
   dfree(e, e->big_tab);
@@ -2758,15 +2820,47 @@ int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new) {
 
 namespace {
 
-// Adds the kernel times of batch slot i (events of launch_batch) to the profile.
-void prof_batch(usap_engine* e, size_t i) {
-  const int kid[3] = {USAP_K_NODAL, USAP_K_APPLY, USAP_K_GROW};
-  for (int k = 0; k < 3; ++k) {
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e->bev[6 * i + 2 * k], e->bev[6 * i + 2 * k + 1]);
-    e->kp[kid[k]].ms += ms;
-    e->kp[kid[k]].launches += 1;
+// Profiling of a round of batch launches (slot i = the round's batch i, of
+// at most `batch` nodes): every workgroup of k_nodal, k_apply and k_grow
+// stamps its start and end, and each kernel's time is its first workgroup
+// start to its last workgroup end, the kernel's own execution.  Events on
+// these launches would add cache flushes between the kernels and slow them.
+int prof_begin(usap_engine* e, size_t nslots, int32_t batch) {
+  const int32_t G = (int32_t)blocks_for(batch, kNodalWaves);
+  const size_t n = nslots * (4 * (size_t)G + 2 * kGrowBlocks) + 6 * nslots;
+  if (e->tstamp_n < n) {
+    dfree(e, e->tstamp);
+    if (int rc = dalloc(e, &e->tstamp, n)) return rc;
+    e->tstamp_n = n;
   }
+  e->tstamp_G = G;
+  k_stamp_init<<<std::min<unsigned>(blocks_for((int64_t)n, 256), 4096u), 256, 0, e->stream>>>(
+      e->tstamp, (int64_t)n);
+  HIPCHK(hipGetLastError());
+  return USAP_OK;
+}
+
+// Adds the kernel times of the round's first `executed` batch slots to the
+// profile (after the round's launches).
+int prof_end(usap_engine* e, size_t executed) {
+  if (executed == 0) return USAP_OK;
+  const int32_t G = e->tstamp_G;
+  unsigned long long* out = e->tstamp + e->tstamp_n - 6 * executed;  // past every slot's stamps
+  k_stamp_reduce<<<(unsigned)(3 * executed), 256, 0, e->stream>>>(e->tstamp, G, kGrowBlocks, out);
+  HIPCHK(hipGetLastError());
+  std::vector<unsigned long long> t(6 * executed);
+  HIPCHK(hipMemcpyAsync(t.data(), out, sizeof(unsigned long long) * 6 * executed,
+                        hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const int kid[3] = {USAP_K_NODAL, USAP_K_APPLY, USAP_K_GROW};
+  for (size_t i = 0; i < executed; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      const unsigned long long t0 = t[6 * i + 2 * k], t1 = t[6 * i + 2 * k + 1];
+      if (t1 > t0) e->kp[kid[k]].ms += 1e-5 * (double)(t1 - t0);  // 100 MHz clock
+      e->kp[kid[k]].launches += 1;
+    }
+  }
+  return USAP_OK;
 }
 
 // Launches k_nodal + k_apply + k_grow for nodes [beg, end), batch bidx of the
@@ -2788,6 +2882,13 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   a.exact = e->p.ds_mode == USAP_DS_EXACT;
   a.seed = e->p.seed;
   a.bidx = bidx;
+  // profiled: this slot's workgroup stamps (k_nodal, k_apply: G starts + G
+  // ends each; k_grow: kGrowBlocks + kGrowBlocks)
+  const int32_t G = e->tstamp_G;
+  unsigned long long* ts =
+      e->prof ? e->tstamp + ev_slot * (4 * (size_t)G + 2 * kGrowBlocks) : nullptr;
+  a.tstamp = ts;
+  a.tstride = G;
   a.big_off = e->big_off;
   a.big_tab = e->big_tab;
   a.prop_s = e->prop_s;
@@ -2820,6 +2921,8 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   b.part = e->part;
   b.status = e->status;
   b.sweep = e->sweep;
+  b.tstamp = ts ? ts + 2 * (size_t)G : nullptr;
+  b.tstride = G;
   const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
   GrowArgs ga;
   DevSpM* ms[2] = {&e->M, &e->Mt};
@@ -2842,22 +2945,13 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   ga.part = e->part;
   ga.nparts = grid;
   ga.status = e->status;
-  if (e->prof) {
-    // start/stop events attached to each launch time the kernel itself
-    while (e->bev.size() < 6 * (ev_slot + 1)) {
-      hipEvent_t ev;
-      HIPCHK(hipEventCreate(&ev));
-      e->bev.push_back(ev);
-    }
-    hipEvent_t* ev = &e->bev[6 * ev_slot];
-    hipExtLaunchKernelGGL(k_nodal, dim3(grid), dim3(kWave * kNodalWaves), 0, st, ev[0], ev[1], 0, a);
-    hipExtLaunchKernelGGL(k_apply, dim3(grid), dim3(kWave * kNodalWaves), 0, st, ev[2], ev[3], 0, b);
-    hipExtLaunchKernelGGL(k_grow, dim3(128), dim3(256), 0, st, ev[4], ev[5], 0, ga);
-  } else {
-    k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
-    k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
-    k_grow<<<128, 256, 0, st>>>(ga);
-  }
+  ga.tstamp = ts ? ts + 4 * (size_t)G : nullptr;
+  ga.tstride = kGrowBlocks;
+  if (e->prof && (int64_t)grid > G)
+    return set_err(e, USAP_ESTATE, "profiled batch larger than its stamp slot");
+  k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
+  k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
+  k_grow<<<kGrowBlocks, 256, 0, st>>>(ga);
   HIPCHK(hipGetLastError());
   return USAP_OK;
 }
@@ -2917,6 +3011,8 @@ int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, int32_
     if (e->p.graphs && !e->prof && next == 0) {
       if (int rc = sweep_graph(e, batch, last)) return rc;
     } else {
+      if (e->prof)
+        if (int rc = prof_begin(e, (size_t)(last - next), batch)) return rc;
       size_t slot = 0;
       for (int32_t i = next; i < last; ++i, ++slot) {
         const int32_t beg = i * batch;
@@ -2927,9 +3023,8 @@ int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, int32_
     HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const int32_t executed = first + hs.done - next;  // batches run in this launch round
-    if (e->prof) {
-      for (int32_t i = 0; i < executed; ++i) prof_batch(e, i);
-    }
+    if (e->prof)
+      if (int rc = prof_end(e, (size_t)executed)) return rc;
     if (hs.stop == 3) return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
     if (hs.flags & (kFlagLoad | kFlagFull)) {
       if (int rc = rebuild_M(e)) return rc;
@@ -2975,12 +3070,14 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
     HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
     // one batch of size end-begin starting at begin: index 0 of a sweep whose
     // "batch" starts at begin
+    if (e->prof)
+      if (int rc = prof_begin(e, 1, end - begin)) return rc;
     if (int rc = launch_batch(e, begin, end, 0, 0)) return rc;
     Sweep& hs = *e->h_sweep;
     HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) {
-      prof_batch(e, 0);
+      if (int rc = prof_end(e, 1)) return rc;
       e->kp[USAP_K_NODAL].bytes += (double)hs.nodal_bytes;
       e->kp[USAP_K_APPLY].bytes += (double)hs.apply_bytes;
     }
