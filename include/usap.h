@@ -73,7 +73,7 @@ typedef struct usap_params {
   uint64_t seed;                   /* random-stream key */
   int32_t device;                  /* HIP device ordinal */
   int32_t verbose;                 /* print per-phase progress to stderr */
-  int32_t slab_slack;              /* hashed M row capacity = pow2 >= slack*edges+16 (4; 1..16) */
+  int32_t slab_slack;              /* hashed block row capacity = pow2 >= slack*(out+in edges)+16; a row grows past 1/slack load (2; 1..16) */
   int32_t graphs;                  /* 1: replay each nodal sweep as a captured HIP graph (0 =
                                       default; only with one engine per process: another
                                       thread's HIP calls can invalidate a capture) */

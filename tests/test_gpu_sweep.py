@@ -70,13 +70,15 @@ def test_sweep_equals_batch_loop(usap, orc, mode, slack):
         stream += a["batches"]
 
 
-@pytest.mark.parametrize("gname", ["c1", "weighted"])
+@pytest.mark.parametrize("gname", ["sparse", "weighted"])
 def test_partition_with_tight_slabs(usap, orc, gname):
     """slab_slack=1 makes hashed rows fill up often: rows are moved to larger
     slabs between batches (and M is rebuilt when the pool runs out); the
-    partition must not change."""
-    g = usap.config_graph("c1") if gname == "c1" else random_graph(
-        700, 9000, seed=5, weighted=True, self_loops=True, hubs=2, isolated=7)
+    partition must not change.  The sparse graph keeps B large against the
+    rows' key counts (a row is direct once 8 x its slab reaches B), so its
+    rows stay hashed."""
+    g = random_graph(3000, 24000, seed=6, hubs=2) if gname == "sparse" else random_graph(
+        3000, 24000, seed=5, weighted=True, self_loops=True, hubs=2, isolated=7)
     params = dict(seed=2, init=1, ds_mode=1)
     eng = usap.Engine(g, slab_slack=1, **params)
     lab, st = eng.partition()

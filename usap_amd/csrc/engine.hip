@@ -1,17 +1,22 @@
 // engine.hip -- MI355X (gfx950) engine for uSAP's partition hot path.
 //
 // Device state per graph: CSR/CSC (int64 offsets, int32 ids), labels int32[N],
-// block degrees int32[B], and the interblock edge-count matrix M kept twice
-// (rows of M and rows of M^T = columns of M) in one slab pool per orientation:
+// block degrees int32[B], and the interblock edge-count matrix M as one "block
+// row" per block a holding both orientations: the slot of key b carries
+// o = M[a][b] (edges a -> b) and i = M[b][a] (edges b -> a), so every M[a][b]
+// is stored twice (as o in row a and as i in row b) and one probe of row a
+// answers both the row and the column lookup of a block pair.  Rows live in
+// one slab pool:
 //   row a occupies slots [off[a], off[a]+cap[a]);
-//   cap[a] == B  -> direct-mapped row (slot = column), used when a hashed row
+//   cap[a] == B  -> direct-mapped row (slot = key), used when a hashed row
 //                   would need >= B slots (dense regime, small B);
 //   cap[a] <  B  -> open-addressing hash (power of two, linear probing);
-//                   a slot is (key, count), int32 each (key -1 = empty), so a
-//                   probe and the count update after it share a cache line.
-// Keys are never deleted inside a sweep (a count may drop to 0).  Each row
-// also keeps kNBK bucket sums (sum of its counts per key range) for the
-// two-level inverse-CDF pick.  A row whose key count passes 1/4 of its slab
+//                   a slot is 16 bytes (key, -, o, i; key -1 = empty), so a
+//                   probe and the count update after it share a cache line
+//                   and one 64-bit atomic updates both counts.
+// Keys are never deleted inside a sweep (counts may drop to 0).  Each row
+// also keeps kNBK bucket sums (sum of o + i per key range) for the two-level
+// inverse-CDF pick over the combined weights M[a][b] + M[b][a].  A row whose key count passes 1/4 of its slab
 // during nodal batches is queued; an increment that finds no free slot is parked in an overflow
 // buffer; between two batches k_grow compacts queued rows in place or moves
 // them to a larger slab at the pool's tail and replays the parked increments.
@@ -68,7 +73,7 @@ constexpr int kNodalWaves = 4;           // wavefronts per workgroup in k_nodal 
 constexpr int kTab = 512;                // per-wavefront LDS table slots
 constexpr int kTabMaxDeg = kTab * 3 / 4; // larger nodes use a global-memory table
 constexpr int kMergeWaves = 4;           // wavefronts per workgroup in k_merge
-constexpr int kNBK = 512;                // bucket sums per row of M and M^T
+constexpr int kNBK = 512;                // bucket sums per block row
 constexpr int kScr = 3 * kTab;           // per-wavefront LDS scratch (ints) for one bucket
 constexpr int kDegLds = 1024;            // k_degrees privatises its counters for B <= this
 constexpr int kSmallB = 128;             // k_insert_small: B*B counts in LDS
@@ -116,28 +121,35 @@ struct GraphView {
   const int32_t* in_w;
 };
 
-// One slot of a row slab: key (-1 empty; unused in a direct row) and count
-// side by side, so a probe and the count update that follows it touch one
-// cache line.
-struct __align__(8) Slot {
-  int32_t k, v;
+// One slot of a block row slab: key (-1 empty; unused in a direct row) and
+// the pair's two counts, o = M[a][key] and i = M[key][a].  (o, i) is 8-byte
+// aligned: one 64-bit atomic adds to both.
+struct __align__(16) Slot {
+  int32_t k, pad;
+  int32_t o, i;
 };
 
-// One orientation of M, as seen by kernels.
+// The two counts of a block pair (a, b): o = M[a][b], i = M[b][a].
+struct Cnt {
+  int32_t o, i;
+};
+
+// M as block rows, as seen by kernels.
 struct MView {
   const int64_t* off;
   const int32_t* cap;
   int32_t* nkeys;
   Slot* kv;        // row slabs
-  int32_t* bs;     // [B][kNBK]: sum of the row's counts per key bucket
+  int32_t* bs;     // [B][kNBK]: sum of the row's o + i per key bucket
   int32_t* gmark;  // [B]: row queued for growth
   int32_t* gq;     // [B]: growth queue
   int32_t* gpend;  // [B]: increments of the row waiting in the overflow buffer
-  int32_t* ovf;    // [3][ovf_cap]: (row, key, count) increments that found no free slot
+  int32_t* ovf;    // [4][ovf_cap]: (row, key, do, di) increments that found no free slot
   int32_t ovf_cap;
   struct GrowState* gst;
   int32_t B;
   int32_t bshift;  // bucket of key k = k >> bshift
+  int32_t slack;   // slab_slack: a hashed row is queued for growth past 1/slack load
 };
 
 // Rows to grow between two batches; slabs for grown rows come from the free
@@ -180,7 +192,7 @@ __device__ __forceinline__ int64_t m_slot(const MView& m, Row R, int32_t b) {
 // of its lookups in flight at once instead of one memory latency per lookup.
 struct Probe {
   int64_t slot;
-  int32_t k, v;
+  int32_t k, o, i;
 };
 
 __device__ __forceinline__ Probe m_probe(const MView& m, Row R, int32_t b) {
@@ -188,68 +200,84 @@ __device__ __forceinline__ Probe m_probe(const MView& m, Row R, int32_t b) {
   p.slot = m_slot(m, R, b);
   const Slot q = m.kv[p.slot];
   p.k = q.k;
-  p.v = q.v;
+  p.o = q.o;
+  p.i = q.i;
   return p;
 }
 
 // Linear probing on from slot `slot` of a hashed row after a collision there.
-__device__ __forceinline__ int32_t m_value_more(const Slot* kv, int64_t o,
-                                             int32_t c, int32_t b, int64_t slot) {
+__device__ __forceinline__ Cnt m_value_more(const Slot* kv, int64_t o, int32_t c, int32_t b,
+                                            int64_t slot) {
   const uint32_t mask = (uint32_t)c - 1;
   uint32_t h = (uint32_t)(slot - o);
   for (int32_t probe = 1; probe < c; ++probe) {
     h = (h + 1) & mask;
     const Slot q = kv[o + h];
-    const int32_t k = q.k, v = q.v;
-    if (k == b) return v;
-    if (k < 0) return 0;
+    if (q.k == b) return Cnt{q.o, q.i};
+    if (q.k < 0) return Cnt{0, 0};
   }
-  return 0;
+  return Cnt{0, 0};
 }
 
-// M[a][b] from the first probe p of key b in row R of a.
-__device__ __forceinline__ int32_t m_value(const MView& m, Row R, int32_t b, const Probe& p) {
-  if (R.c == m.B || p.k == b) return p.v;
-  if (p.k < 0) return 0;
+// (M[a][b], M[b][a]) from the first probe p of key b in row R of a.
+__device__ __forceinline__ Cnt m_value(const MView& m, Row R, int32_t b, const Probe& p) {
+  if (R.c == m.B || p.k == b) return Cnt{p.o, p.i};
+  if (p.k < 0) return Cnt{0, 0};
   return m_value_more(m.kv, R.o, R.c, b, p.slot);
 }
 
-// M[a][b]
-__device__ __forceinline__ int32_t m_get(const MView& m, int32_t a, int32_t b) {
+// (M[a][b], M[b][a])
+__device__ __forceinline__ Cnt m_get(const MView& m, int32_t a, int32_t b) {
   const Row R = m_row(m, a);
   return m_value(m, R, b, m_probe(m, R, b));
 }
 
-// M[a][b] += w in row R of a (w < 0 only for a key that exists), given the
-// key k read at the first probed slot `slot`.  Keys only go from empty to
-// set, so an earlier read is a valid first read: a stale "empty" makes the
-// claim fail and return the key that is there.  kBuckets also updates the
-// row's bucket sum and queues rows for growth instead of flagging a rebuild.
+// o += dout, i += din with one 64-bit atomic.  Exact as long as neither count
+// leaves [0, 2^31) at any point, which holds for every caller: a decrement
+// only removes edges the count held when the kernel started.
+__device__ __forceinline__ void cnt_add(Slot* q, int32_t dout, int32_t din) {
+  const long long d = (long long)din * 4294967296LL + (long long)dout;
+  atomicAdd(reinterpret_cast<unsigned long long*>(&q->o), (unsigned long long)d);
+}
+
+constexpr int32_t kMaxWalk = 16;  // probes before a nodal-batch increment is parked
+
+// M[a][b] += dout and M[b][a] += din in row R of a (negative only for a key
+// that exists), given the key k read at the first probed slot `slot`.  Keys
+// only go from empty to set, so an earlier read is a valid first read: a
+// stale "empty" makes the claim fail and return the key that is there.
+// kBuckets also updates the row's bucket sum and queues rows for growth
+// instead of flagging a rebuild.
 template <bool kBuckets>
-__device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32_t b, int32_t w,
-                                         int64_t slot, int32_t k, int32_t* flags) {
-  if (kBuckets) atomicAdd(&m.bs[(int64_t)a * kNBK + (b >> m.bshift)], w);
+__device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32_t b, int32_t dout,
+                                         int32_t din, int64_t slot, int32_t k, int32_t* flags) {
+  if (kBuckets && dout + din != 0)
+    atomicAdd(&m.bs[(int64_t)a * kNBK + (b >> m.bshift)], dout + din);
   if (R.c == m.B || k == b) {
-    atomicAdd(&m.kv[slot].v, w);
+    cnt_add(&m.kv[slot], dout, din);
     return;
   }
+  const bool dec = dout < 0 || din < 0;
   const int64_t o = R.o;
   const int32_t c = R.c;
   const uint32_t mask = (uint32_t)c - 1;
   uint32_t h = (uint32_t)(slot - o);
   for (int32_t probe = 0; probe < c; ++probe) {
+    // an increment whose key is still absent after kMaxWalk slots is parked
+    // (the row is queued for growth) rather than walking a crowded row
+    if (kBuckets && !dec && probe >= kMaxWalk) break;
     if (probe > 0) k = ld_relaxed(&m.kv[o + h].k);
     if (k < 0) {
-      if (w < 0) {
+      if (dec) {
         atomicOr(flags, kFlagMissing);
         return;
       }
       k = atomicCAS(&m.kv[o + h].k, -1, b);
       if (k == -1) {
         const int32_t n = atomicAdd(&m.nkeys[a], 1) + 1;
-        // past 1/4 load during nodal batches (short probe chains for the
-        // lookups, most of which miss) or 3/4 in a build: grow / rebuild
-        if (kBuckets ? 4 * n > c : 4 * n > 3 * c) {
+        // past 1/slack load during nodal batches (short probe chains for
+        // the lookups, most of which miss) or 3/4 in a build: grow / rebuild
+        if (kBuckets ? m.slack * n > c : 4 * n > 3 * c) {
           if (!kBuckets) {
             atomicOr(flags, kFlagLoad);
           } else if (atomicCAS(&m.gmark[a], 0, 1) == 0) {  // queue the row once
@@ -260,40 +288,65 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
       }
     }
     if (k == b) {
-      atomicAdd(&m.kv[o + h].v, w);
+      cnt_add(&m.kv[o + h], dout, din);
       return;
     }
     h = (h + 1) & mask;
   }
-  if (kBuckets && w > 0) {  // slab full: park the increment until k_grow has grown the row
+  if (kBuckets && !dec) {  // no free slot near: park the increment until k_grow has grown the row
     atomicAdd(&m.gst->full, 1ull);
     const int32_t i = atomicAdd(&m.gst->ovf_n, 1);
     if (i < m.ovf_cap) {
       m.ovf[i] = a;
       m.ovf[m.ovf_cap + i] = b;
-      m.ovf[2 * m.ovf_cap + i] = w;
+      m.ovf[2 * m.ovf_cap + i] = dout;
+      m.ovf[3 * m.ovf_cap + i] = din;
       atomicAdd(&m.gpend[a], 1);
       if (atomicCAS(&m.gmark[a], 0, 1) == 0) m.gq[atomicAdd(&m.gst->n, 1)] = a;
       return;
     }
   }
-  atomicOr(flags, w < 0 ? kFlagMissing : kFlagFull);
+  atomicOr(flags, dec ? kFlagMissing : kFlagFull);
 }
 
-// First key of b in row R, for m_add_at.
+// Where m_add_at starts for key b in row R: the keys of kWin slots from b's
+// home slot are read at once (one memory latency for most probe chains) and
+// the first that holds b or is empty is returned with its slot; if none does,
+// the window's last slot, from which m_add_at walks on.
+constexpr int kWin = 8;
 __device__ __forceinline__ int32_t m_first_key(const MView& m, Row R, int32_t b, int64_t& slot) {
-  slot = m_slot(m, R, b);
-  return R.c == m.B ? b : ld_relaxed(&m.kv[slot].k);
+  if (R.c == m.B) {
+    slot = R.o + b;
+    return b;
+  }
+  const uint32_t mask = (uint32_t)R.c - 1;  // hashed rows hold >= 32 slots
+  const uint32_t h = hash32(b) & mask;
+  int32_t k[kWin];
+#pragma unroll
+  for (int q = 0; q < kWin; ++q) k[q] = ld_relaxed(&m.kv[R.o + ((h + q) & mask)].k);
+  int32_t kk = k[kWin - 1];
+  uint32_t pos = (h + kWin - 1) & mask;
+  bool found = false;
+#pragma unroll
+  for (int q = 0; q < kWin; ++q) {
+    if (!found && (k[q] == b || k[q] < 0)) {
+      found = true;
+      kk = k[q];
+      pos = (h + q) & mask;
+    }
+  }
+  slot = R.o + pos;
+  return kk;
 }
 
-// M[a][b] += w
+// M[a][b] += dout, M[b][a] += din in row a
 template <bool kBuckets>
-__device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int32_t w,
-                                      int32_t* flags) {
+__device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int32_t dout,
+                                      int32_t din, int32_t* flags) {
   const Row R = m_row(m, a);
   int64_t slot;
   const int32_t k = m_first_key(m, R, b, slot);
-  m_add_at<kBuckets>(m, a, R, b, w, slot, k, flags);
+  m_add_at<kBuckets>(m, a, R, b, dout, din, slot, k, flags);
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -371,32 +424,36 @@ __device__ __forceinline__ i128 Tterm(const int64_t* lam, int64_t m, int64_t x, 
   return (m > 0 && x > 0 && y > 0) ? (i128)m * (i128)(Lm - Lx - Ly) : (i128)0;
 }
 
-// per-lane part of sum_{lo <= key <= x, key != excl} M[a][key]
+// per-lane part of sum_{lo <= key <= x, key != excl} M[a][key] + M[key][a]
+// over row (o, c) of a
 __device__ __forceinline__ long count_range_lane(const MView& m, int64_t o, int32_t c, int32_t lo,
                                                  int32_t x, int32_t excl) {
   const int lane = lane_id();
   long acc = 0;
   if (c == m.B) {
     for (int32_t i = lo + lane; i <= x; i += kWave)
-      if (i != excl) acc += m.kv[o + i].v;
+      if (i != excl) acc += (long)m.kv[o + i].o + m.kv[o + i].i;
   } else {
     for (int32_t j = lane; j < c; j += kWave) {
-      const int32_t k = m.kv[o + j].k;
-      if (k >= lo && k <= x && k != excl) acc += m.kv[o + j].v;
+      const Slot q = m.kv[o + j];
+      if (q.k >= lo && q.k <= x && q.k != excl) acc += (long)q.o + q.i;
     }
   }
   return acc;
 }
 
-// scr[k - lo] += M[a][k] for lo <= k < hi, from row R of a: a direct row is
-// read at columns [lo, hi), a hashed row's slab is scanned eight slots per
-// lane at a time (all loads of a pass in flight before the LDS adds).
+// scr[k - lo] += M[a][k] + M[k][a] for lo <= k < hi, from row R of a: a
+// direct row is read at keys [lo, hi), a hashed row's slab is scanned eight
+// slots per lane at a time (all loads of a pass in flight before the LDS adds).
 __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, int32_t hi,
                                            int32_t* scr, long& bytes) {
   const int lane = lane_id();
   if (R.c == m.B) {
-    for (int32_t i = lane; i < hi - lo; i += kWave) atomicAdd(&scr[i], m.kv[R.o + lo + i].v);
-    bytes += 4 * (long)(hi - lo);
+    for (int32_t i = lane; i < hi - lo; i += kWave) {
+      const Slot q = m.kv[R.o + lo + i];
+      atomicAdd(&scr[i], q.o + q.i);
+    }
+    bytes += 8 * (long)(hi - lo);
     return;
   }
   for (int32_t base = 0; base < R.c; base += 8 * kWave) {
@@ -409,48 +466,14 @@ __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, in
       if (j < R.c) {
         const Slot x = m.kv[R.o + j];
         k[q] = x.k;
-        v[q] = x.v;
+        v[q] = x.o + x.i;
       }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       if (k[q] >= lo && k[q] < hi && v[q] != 0) atomicAdd(&scr[k[q] - lo], v[q]);
   }
-  bytes += 8 * (long)R.c;
-}
-
-// gather_row for row u of M and row u of M^T together: the passes over both
-// slabs are interleaved so their loads are in flight at the same time.
-__device__ __forceinline__ void gather_rows(const MView& M, Row RM, const MView& Mt, Row RT,
-                                            int32_t lo, int32_t hi, int32_t* scr, long& bytes) {
-  if (RM.c == M.B || RT.c == Mt.B) {  // a direct row: its range is one short pass
-    gather_row(M, RM, lo, hi, scr, bytes);
-    gather_row(Mt, RT, lo, hi, scr, bytes);
-    return;
-  }
-  const int lane = lane_id();
-  const int32_t cmax = max(RM.c, RT.c);
-  for (int32_t base = 0; base < cmax; base += 4 * kWave) {
-    int32_t k[8], v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const bool t = q >= 4;
-      const Row& R = t ? RT : RM;
-      const Slot* kv = t ? Mt.kv : M.kv;
-      const int32_t j = base + (q & 3) * kWave + lane;
-      k[q] = -1;
-      v[q] = 0;
-      if (j < R.c) {
-        const Slot x = kv[R.o + j];
-        k[q] = x.k;
-        v[q] = x.v;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      if (k[q] >= lo && k[q] < hi && v[q] != 0) atomicAdd(&scr[k[q] - lo], v[q]);
-  }
-  bytes += 8 * ((long)RM.c + RT.c);
+  bytes += 12 * (long)R.c;
 }
 
 // Index of the t-th unit in the LDS weights scr[0, width) (t < their sum):
@@ -481,37 +504,38 @@ __device__ __forceinline__ int32_t scan_scratch(const int32_t* scr, int32_t widt
   return __shfl(res, f);
 }
 
-// Inverse-CDF pick over the combined weights M[u][i] + M[i][u] (row u of M
-// and row u of M^T), ascending i, skipping i == excl: the first i whose
-// running sum exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact
-// integer weights).  The caller guarantees t < total weight.  Two levels: the
-// rows' bucket sums (all loaded at once) locate the key bucket [lo, hi); the
-// bucket's weights then come from one load per key when both rows are direct
-// and the bucket is at most a wavefront wide, else they are gathered into the
+// Inverse-CDF pick over the combined weights M[u][i] + M[i][u] (block row
+// u: o + i), ascending i, skipping i == excl: the first i whose running sum
+// exceeds t (graph_parallel.hpp:849-850 / :1061-1062 with exact integer
+// weights).  The caller guarantees t < total weight.  Two levels: the row's
+// bucket sums (all loaded at once) locate the key bucket [lo, hi); the
+// bucket's weights then come from one load per key when the row is direct and
+// the bucket is at most a wavefront wide, else they are gathered into the
 // wavefront's LDS scratch `scr` (scr_cap ints) -- a hashed row by scanning its
 // slab, which costs a few loads per lane where looking up each absent key
 // would walk a probe chain -- and searched with one scan.
-__device__ __forceinline__ int32_t select_combined(const MView& M, const MView& Mt, int32_t u, int64_t t,
-                                   int32_t excl, int32_t* scr, int32_t scr_cap, long& bytes) {
+__device__ __forceinline__ int32_t select_combined(const MView& M, int32_t u, int64_t t,
+                                                   int32_t excl, int32_t* scr, int32_t scr_cap,
+                                                   long& bytes) {
   const int lane = lane_id();
   const int32_t B = M.B;
   const int sh = M.bshift;
-  const Row RM = m_row(M, u), RT = m_row(Mt, u);
+  const Row RM = m_row(M, u);
   const int32_t nbk = ((B - 1) >> sh) + 1;
   const int32_t* bM = M.bs + (int64_t)u * kNBK;
-  const int32_t* bT = Mt.bs + (int64_t)u * kNBK;
   constexpr int kQ = kNBK / kWave;
   long wq[kQ];
 #pragma unroll
   for (int q = 0; q < kQ; ++q) {
     const int32_t j = q * kWave + lane;
-    wq[q] = j < nbk ? (long)bM[j] + (long)bT[j] : 0;
+    wq[q] = j < nbk ? (long)bM[j] : 0;
   }
-  bytes += 24 + 8 * (long)nbk;
+  bytes += 12 + 4 * (long)nbk;
   if (excl >= 0) {
-    const Probe p = m_probe(M, RM, excl), pt = m_probe(Mt, RT, excl);
-    const long wx = (long)m_value(M, RM, excl, p) + (long)m_value(Mt, RT, excl, pt);
-    bytes += 16;
+    const Probe p = m_probe(M, RM, excl);
+    const Cnt x = m_value(M, RM, excl, p);
+    const long wx = (long)x.o + x.i;
+    bytes += 12;
     const int32_t xb = excl >> sh;
 #pragma unroll
     for (int q = 0; q < kQ; ++q)
@@ -535,23 +559,30 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
   const int32_t lo = jb << sh;
   const int32_t hi = min(B, lo + (1 << sh));
   const int32_t width = hi - lo;
-  if (width <= kWave && RM.c == B && RT.c == B) {  // direct rows: one load per key
+  if (width <= kWave && RM.c == B) {  // direct row: one load per key
     long w = 0;
     const int32_t i = lo + lane;
-    if (lane < width && i != excl) w = (long)M.kv[RM.o + i].v + (long)Mt.kv[RT.o + i].v;
+    if (lane < width && i != excl) {
+      const Slot q = M.kv[RM.o + i];
+      w = (long)q.o + q.i;
+    }
     bytes += 8 * (long)width;
     const long inc = wave_incl_scan(w);
     const int f = __ffsll((unsigned long long)__ballot(inc > t)) - 1;
     return f < 0 ? hi - 1 : lo + f;
   }
   if (width <= scr_cap) {
-    if (RM.c == B && RT.c == B) {
+    if (RM.c == B) {
       for (int32_t base = 0; base < width; base += 4 * kWave) {
         int32_t v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int32_t i = base + q * kWave + lane;
-          v[q] = i < width ? M.kv[RM.o + lo + i].v + Mt.kv[RT.o + lo + i].v : 0;
+          v[q] = 0;
+          if (i < width) {
+            const Slot x = M.kv[RM.o + lo + i];
+            v[q] = x.o + x.i;
+          }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -563,7 +594,7 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
     } else {
       for (int32_t i = lane; i < width; i += kWave) scr[i] = 0;
       wave_fence();
-      gather_rows(M, RM, Mt, RT, lo, hi, scr, bytes);
+      gather_row(M, RM, lo, hi, scr, bytes);
     }
     wave_fence();
     if (lane == 0 && excl >= lo && excl < hi) scr[excl - lo] = 0;
@@ -574,8 +605,7 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, const MView& 
   int32_t a = lo, b = hi - 1;
   while (a < b) {
     const int32_t mid = a + (b - a) / 2;
-    const long c = wave_sum(count_range_lane(M, RM.o, RM.c, lo, mid, excl) +
-                            count_range_lane(Mt, RT.o, RT.c, lo, mid, excl));
+    const long c = wave_sum(count_range_lane(M, RM.o, RM.c, lo, mid, excl));
     if (c > t)
       b = mid;
     else
@@ -668,41 +698,45 @@ __device__ __forceinline__ int32_t cap_for(int32_t B, int32_t ecnt, int32_t slac
   return 8 * c >= B ? B : (int32_t)c;
 }
 
+// Block row a holds the keys of its out-edges and of its in-edges: its slab
+// is sized for ecnt_out[a] + ecnt_in[a] keys.
 __global__ void k_caps(int32_t B, int32_t slack, const int32_t* ecnt_out, const int32_t* ecnt_in,
-                       int32_t* capM, int32_t* capT, int64_t* tmpM, int64_t* tmpT) {
+                       int32_t* cap, int64_t* tmp) {
   const int32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a > B) return;
   if (a == B) {
-    tmpM[B] = 0;
-    tmpT[B] = 0;
+    tmp[B] = 0;
     return;
   }
-  const int32_t cm = cap_for(B, ecnt_out[a], slack);
-  const int32_t ct = cap_for(B, ecnt_in[a], slack);
-  capM[a] = cm;
-  capT[a] = ct;
-  tmpM[a] = cm;
-  tmpT[a] = ct;
+  const int64_t ec = (int64_t)ecnt_out[a] + ecnt_in[a];
+  const int32_t c = cap_for(B, (int32_t)min<int64_t>(ec, (int64_t)B), slack);
+  cap[a] = c;
+  tmp[a] = c;
 }
 
 __global__ void k_fill(Slot* kv, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    kv[i] = Slot{-1, 0};
+    kv[i] = Slot{-1, 0, 0, 0};
   }
 }
 
+// Edge u -> v between blocks a -> b: o of row a at key b and i of row b at
+// key a (one update when a == b).
 __global__ void k_insert_edges(int64_t E, const int32_t* __restrict__ src,
                                const int32_t* __restrict__ dst, const int32_t* __restrict__ w,
-                               const int32_t* __restrict__ labels, MView M, MView Mt,
-                               int32_t* flags) {
+                               const int32_t* __restrict__ labels, MView M, int32_t* flags) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int32_t a = labels[src[e]];
     const int32_t b = labels[dst[e]];
     const int32_t ww = w ? w[e] : 1;
-    m_add<false>(M, a, b, ww, flags);
-    m_add<false>(Mt, b, a, ww, flags);
+    if (a == b) {
+      m_add<false>(M, a, a, ww, ww, flags);
+    } else {
+      m_add<false>(M, a, b, ww, 0, flags);
+      m_add<false>(M, b, a, 0, ww, flags);
+    }
   }
 }
 
@@ -712,7 +746,7 @@ __global__ __launch_bounds__(256) void k_insert_small(int64_t E, const int32_t* 
                                                       const int32_t* __restrict__ dst,
                                                       const int32_t* __restrict__ w,
                                                       const int32_t* __restrict__ labels,
-                                                      MView M, MView Mt) {
+                                                      MView M) {
   __shared__ int32_t s_h[kSmallB * kSmallB];
   const int32_t B = M.B;
   for (int i = threadIdx.x; i < B * B; i += blockDim.x) s_h[i] = 0;
@@ -725,12 +759,12 @@ __global__ __launch_bounds__(256) void k_insert_small(int64_t E, const int32_t* 
     const int32_t v = s_h[i];
     if (!v) continue;
     const int32_t a = i / B, b = i - a * B;
-    atomicAdd(&M.kv[M.off[a] + b].v, v);
-    atomicAdd(&Mt.kv[Mt.off[b] + a].v, v);
+    atomicAdd(&M.kv[M.off[a] + b].o, v);
+    atomicAdd(&M.kv[M.off[b] + a].i, v);
   }
 }
 
-// Bucket sums of every row (one wavefront per row).
+// Bucket sums (o + i) of every block row (one wavefront per row).
 __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   __shared__ int32_t s_b[4][kNBK];
   const int w = threadIdx.x / kWave, lane = lane_id();
@@ -743,8 +777,9 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   const int32_t c = m.cap[a];
   const bool direct = c == m.B;
   for (int32_t j = lane; j < c; j += kWave) {
-    const int32_t k = direct ? j : m.kv[o + j].k;
-    const int32_t v = m.kv[o + j].v;
+    const Slot q = m.kv[o + j];
+    const int32_t k = direct ? j : q.k;
+    const int32_t v = q.o + q.i;
     if (k >= 0 && v != 0) atomicAdd(&bk[k >> m.bshift], v);
   }
   wave_fence();
@@ -833,7 +868,7 @@ __global__ __launch_bounds__(256) void k_entropy(const Slot* __restrict__ kv,
   i128 acc = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
-    const int32_t v = kv[i].v;
+    const int32_t v = kv[i].o;  // each M[a][b] once, as o of row a
     if (v > 0) acc += (i128)v * (i128)lam[v];
   }
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < B; b += stride)
@@ -864,7 +899,7 @@ __global__ void k_sort_hi(int32_t B, const long* __restrict__ hi, const int32_t*
 // ---------------------------------------------------------------------------
 
 struct MergeArgs {
-  MView M, Mt;
+  MView M;
   const int32_t* d_out;
   const int32_t* d_in;
   const int64_t* lam;
@@ -877,22 +912,23 @@ struct MergeArgs {
 };
 
 // per-lane part of sum_{i not in {r,s}} [L(m1) + L(m2) - L(m1+m2)] over the
-// columns where rows r and s of m are both non-zero (the only columns whose
-// terms survive the factorised merge dS).
+// keys where rows r and s of M are both non-zero, and the same over the
+// columns (the only terms that survive the factorised merge dS): one pass
+// over the shorter block row, probing the other for each of its keys.
 __device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r, int32_t s) {
   const int lane = lane_id();
   const bool r_short = m.cap[r] <= m.cap[s];
   const int32_t a1 = r_short ? r : s, a2 = r_short ? s : r;
-  const int64_t o = m.off[a1];
-  const int32_t c = m.cap[a1];
-  const bool direct = c == m.B;
+  const Row R1 = m_row(m, a1), R2 = m_row(m, a2);
+  const bool direct = R1.c == m.B;
   i128 acc = 0;
-  for (int32_t j = lane; j < c; j += kWave) {
-    const int32_t i = direct ? j : m.kv[o + j].k;
-    const int32_t m1 = m.kv[o + j].v;
-    if (i < 0 || m1 <= 0 || i == r || i == s) continue;
-    const int32_t m2 = m_get(m, a2, i);
-    if (m2 > 0) acc += Lterm(lam, m1) + Lterm(lam, m2) - Lterm(lam, (int64_t)m1 + m2);
+  for (int32_t j = lane; j < R1.c; j += kWave) {
+    const Slot q = m.kv[R1.o + j];
+    const int32_t i = direct ? j : q.k;
+    if (i < 0 || (q.o <= 0 && q.i <= 0) || i == r || i == s) continue;
+    const Cnt c2 = m_value(m, R2, i, m_probe(m, R2, i));
+    if (q.o > 0 && c2.o > 0) acc += Lterm(lam, q.o) + Lterm(lam, c2.o) - Lterm(lam, (int64_t)q.o + c2.o);
+    if (q.i > 0 && c2.i > 0) acc += Lterm(lam, q.i) + Lterm(lam, c2.i) - Lterm(lam, (int64_t)q.i + c2.i);
   }
   return acc;
 }
@@ -909,14 +945,15 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
     s = (int32_t)usap::idx_of(x[2], B);
   } else {
     long bytes = 0;
-    const int32_t u = select_combined(a.M, a.Mt, r, usap::idx_of(x[0], k), -1, scr, kScr, bytes);
+    const int32_t u = select_combined(a.M, r, usap::idx_of(x[0], k), -1, scr, kScr, bytes);
     const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
     if (usap::uniform_branch(x[1], da, B)) {
       s = usap::uniform_not_r(x[2], r, B);
     } else {
-      const int64_t W2 = da - m_get(a.M, u, r) - m_get(a.Mt, u, r);
+      const Cnt ur = m_get(a.M, u, r);  // M[u][r], M[r][u]
+      const int64_t W2 = da - ur.o - ur.i;
       s = W2 == 0 ? usap::uniform_not_r(x[2], r, B)
-                  : select_combined(a.M, a.Mt, u, usap::idx_of(x[3], W2), r, scr, kScr, bytes);
+                  : select_combined(a.M, u, usap::idx_of(x[3], W2), r, scr, kScr, bytes);
     }
   }
   s_out = s;
@@ -925,11 +962,12 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
     return;
   }
   const int64_t* lam = a.lam;
-  const int64_t Mrr = m_get(a.M, r, r), Mrs = m_get(a.M, r, s);
-  const int64_t Msr = m_get(a.M, s, r), Mss = m_get(a.M, s, s);
+  const Cnt crs = m_get(a.M, r, s);  // M[r][s], M[s][r]
+  const int64_t Mrr = m_get(a.M, r, r).o, Mrs = crs.o;
+  const int64_t Msr = crs.i, Mss = m_get(a.M, s, s).o;
   const int64_t dos = a.d_out[s], dis = a.d_in[s];
   const int64_t dos2 = dos + dor, dis2 = dis + dir;
-  i128 acc = merge_overlap_lane(a.M, lam, r, s) + merge_overlap_lane(a.Mt, lam, r, s);
+  i128 acc = merge_overlap_lane(a.M, lam, r, s);
   acc = wave_sum128(acc);
   bool inf = false;
   const i128 old_c = Tterm(lam, Mrr, dir, dor, inf) + Tterm(lam, Mrs, dis, dor, inf) +
@@ -1008,7 +1046,7 @@ struct BlockPart {
 
 struct NodalArgs {
   GraphView g;
-  MView M, Mt;
+  MView M;
   const int32_t* labels;
   const int32_t* d_out;
   const int32_t* d_in;
@@ -1146,43 +1184,36 @@ __device__ __forceinline__ void list_entry(const GraphView& g, int64_t o0, int64
 // neighbour-block table.  All lookups of a pass are issued before any is
 // resolved.
 __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, int32_t r,
-                                            int32_t s, Row rM, Row rT, int64_t dor, int64_t dir,
+                                            int32_t s, Row rM, int64_t dor, int64_t dir,
                                             int64_t kc, int64_t k_out, int64_t k_in, long wself,
                                             bool& inf, long& bytes) {
   const int lane = lane_id();
   const int64_t* lam = a.lam;
-  // s's rows and degrees; the table is compacted while they load
-  const Row sM = m_row(a.M, s), sT = m_row(a.Mt, s);
+  // s's row and degrees; the table is compacted while they load
+  const Row sM = m_row(a.M, s);
   const int64_t dos = a.d_out[s], dis = a.d_in[s];
   long wor = 0, wir = 0, wos = 0, wis = 0;
   const int32_t nent = tab_compact(T, r, s, wor, wir, wos, wis);
-  // corner: rows r, s at columns r, s
+  // corner: (r, r), (r, s) with (s, r) beside it, (s, s)
   const Probe pRR = m_probe(a.M, rM, r), pRS = m_probe(a.M, rM, s);
-  const Probe pSR = m_probe(a.M, sM, r), pSS = m_probe(a.M, sM, s);
-  // changed entries of rows r, s (out-edges) and columns r, s (in-edges)
+  const Probe pSS = m_probe(a.M, sM, s);
+  // changed entries of rows r, s (out-edges, o) and columns r, s (in-edges,
+  // i): one probe of block row r and one of s per neighbour block
   i128 acc = 0;
   long lookups = 0;
   bool neg = false;
   for (int32_t j = lane; j < nent; j += kWave) {
     const int32_t b = T.tk[j], wo = T.two[j], wi = T.twi[j];
-    Probe p0{}, p1{}, p2{}, p3{};
+    const Probe p0 = m_probe(a.M, rM, b), p1 = m_probe(a.M, sM, b);
+    const Cnt cr = m_value(a.M, rM, b, p0), cs = m_value(a.M, sM, b, p1);
+    lookups += 2;
     if (wo > 0) {
-      p0 = m_probe(a.M, rM, b);
-      p1 = m_probe(a.M, sM, b);
-    }
-    if (wi > 0) {
-      p2 = m_probe(a.Mt, rT, b);
-      p3 = m_probe(a.Mt, sT, b);
-    }
-    if (wo > 0) {
-      const int64_t Rr = m_value(a.M, rM, b, p0), Rs = m_value(a.M, sM, b, p1);
-      lookups += 2;
+      const int64_t Rr = cr.o, Rs = cs.o;
       neg |= Rr - wo < 0;
       acc += Lterm(lam, Rr) + Lterm(lam, Rs) - Lterm(lam, Rr - wo) - Lterm(lam, Rs + wo);
     }
     if (wi > 0) {
-      const int64_t Cr = m_value(a.Mt, rT, b, p2), Cs = m_value(a.Mt, sT, b, p3);
-      lookups += 2;
+      const int64_t Cr = cr.i, Cs = cs.i;
       neg |= Cr - wi < 0;
       acc += Lterm(lam, Cr) + Lterm(lam, Cs) - Lterm(lam, Cr - wi) - Lterm(lam, Cs + wi);
     }
@@ -1200,10 +1231,12 @@ __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, 
   const long Wo = k_out - wor - wos, Wi = k_in - wir - wis;
   if (__ballot(wself != 0)) wself = wave_sum(wself);  // self-loops are rare
   // neighbour ids + labels (+ weights); rows (offset + capacity) and degrees
-  // of r and s; 2 lookups (key + count) per changed entry and 4 corner lookups
-  bytes += (a.g.out_w ? 12 : 8) * kc + 48 + 16 + 8 * (wave_sum(lookups) + 4);
-  const int64_t Mrr = m_value(a.M, rM, r, pRR), Mrs = m_value(a.M, rM, s, pRS);
-  const int64_t Msr = m_value(a.M, sM, r, pSR), Mss = m_value(a.M, sM, s, pSS);
+  // of r and s; 2 block-row lookups (key + two counts) per changed entry and
+  // 3 corner lookups
+  bytes += (a.g.out_w ? 12 : 8) * kc + 24 + 16 + 12 * (wave_sum(lookups) + 3);
+  const Cnt crs = m_value(a.M, rM, s, pRS);
+  const int64_t Mrr = m_value(a.M, rM, r, pRR).o, Mrs = crs.o;
+  const int64_t Msr = crs.i, Mss = m_value(a.M, sM, s, pSS).o;
   const int64_t dor2 = dor - k_out, dos2 = dos + k_out, dir2 = dir - k_in, dis2 = dis + k_in;
   int64_t rr2, rs2, sr2, ss2;
   if (!a.exact) {  // graph_parallel.hpp:1117-1145: only out-edges update the rows
@@ -1253,7 +1286,7 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
   uint64_t x[4];
   usap::philox4x64((uint64_t)n, 0, a.sweep->stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0,
                    x);
-  const Row rM = m_row(a.M, r), rT = m_row(a.Mt, r);
+  const Row rM = m_row(a.M, r);
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   int64_t k_out = kco, k_in = kci;
   if (g.out_w) {
@@ -1317,14 +1350,14 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
     if (usap::uniform_branch(x[1], da, B))
       s = (int32_t)usap::idx_of(x[2], B);
     else
-      s = select_combined(a.M, a.Mt, u, usap::idx_of(x[3], da), -1, scr, kScrNodal, bytes);
+      s = select_combined(a.M, u, usap::idx_of(x[3], da), -1, scr, kScrNodal, bytes);
   }
   i128 dS = 0;
   bool inf = false;
   int32_t acc = 0;
   if (s != r) {
     wave_fence();  // the table is complete
-    dS = nodal_delta(a, T, r, s, rM, rT, dor, dir, kc, k_out, k_in, wself, inf, bytes);
+    dS = nodal_delta(a, T, r, s, rM, dor, dir, kc, k_out, k_in, wself, inf, bytes);
     acc = (inf || dS < 0) ? 1 : 0;
   }
   if (lane == 0) {
@@ -1345,9 +1378,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
   if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
     a.M.gst->n = 0;
-    a.Mt.gst->n = 0;
     a.M.gst->ovf_n = 0;
-    a.Mt.gst->ovf_n = 0;
   }
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
@@ -1371,13 +1402,13 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Apply accepted moves: M, M^T, degrees, block sizes, batch sum
+// Apply accepted moves: block rows of M, degrees, block sizes, batch sum
 // (graph_parallel.hpp:433-443; equal to the rebuild from the new labels)
 // ---------------------------------------------------------------------------
 
 struct ApplyArgs {
   GraphView g;
-  MView M, Mt;
+  MView M;
   int32_t* labels;
   int32_t* d_out;
   int32_t* d_in;
@@ -1439,11 +1470,12 @@ __device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_
 
 // Moves accepted node n from block r to s: its edges' counts leave row and
 // column r for row and column s.  Edges to neighbours that stay put are first
-// summed per neighbour block in the node's table, so each changed count gets
-// one atomic per node; then the first keys of all eight changed entries of a
-// table entry are read together before the adds.  Old labels of nodes inside
-// the batch come from prop_r, so writing labels cannot race with their reads.
-// Returns the algorithmic bytes.
+// summed per neighbour block b in the node's table (wo = edges n -> b, wi =
+// edges b -> n), so each changed pair of counts gets one 64-bit atomic per
+// node: block row r at key b (-wo, -wi), row s at b (+wo, +wi), row b at r
+// (-wi, -wo) and at s (+wi, +wo); the four first keys are read together
+// before the adds.  Old labels of nodes inside the batch come from prop_r, so
+// writing labels cannot race with their reads.  Returns the algorithmic bytes.
 template <bool kBig>
 __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t r, int32_t s,
                                           int32_t* tab) {
@@ -1453,7 +1485,7 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
   const int64_t o0 = g.out_ptr[n], i0 = g.in_ptr[n];
   const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
   const int64_t kc = kco + kci;
-  const Row rM = m_row(a.M, r), rT = m_row(a.Mt, r), sM = m_row(a.M, s), sT = m_row(a.Mt, s);
+  const Row rM = m_row(a.M, r), sM = m_row(a.M, s);
   const Table T = tab_open<kBig>(tab, a.big_off, a.big_tab, n, kc);
   const uint32_t mask = (uint32_t)T.cap - 1;
   long wo_sum = 0, wi_sum = 0;
@@ -1484,10 +1516,10 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
         if (bn[q] == -2) {
           tab_add(T.tk, T.two, mask, bo[q], w);
         } else {  // the neighbour moved in this batch too: bo -> bn
-          m_add<true>(a.M, r, bo[q], -w, flags);
-          m_add<true>(a.Mt, bo[q], r, -w, flags);
-          m_add<true>(a.M, s, bn[q], w, flags);
-          m_add<true>(a.Mt, bn[q], s, w, flags);
+          m_add<true>(a.M, r, bo[q], -w, 0, flags);
+          m_add<true>(a.M, bo[q], r, 0, -w, flags);
+          m_add<true>(a.M, s, bn[q], w, 0, flags);
+          m_add<true>(a.M, bn[q], s, 0, w, flags);
         }
       } else {
         wi_sum += w;
@@ -1501,35 +1533,18 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
   const int32_t nent = tab_compact(T, -2, -2, x0, x1, x2, x3);
   for (int32_t j = lane; j < nent; j += kWave) {
     const int32_t b = T.tk[j], wo = T.two[j], wi = T.twi[j];
-    Row bM{0, 0}, bT{0, 0};
-    if (wi) bM = m_row(a.M, b);
-    if (wo) bT = m_row(a.Mt, b);
-    int64_t sl[8] = {};
-    int32_t k[8] = {};
-    if (wo) {
-      k[0] = m_first_key(a.M, rM, b, sl[0]);   // M[r][b]  -= wo
-      k[1] = m_first_key(a.Mt, bT, r, sl[1]);  // Mt[b][r] -= wo
-      k[2] = m_first_key(a.M, sM, b, sl[2]);   // M[s][b]  += wo
-      k[3] = m_first_key(a.Mt, bT, s, sl[3]);  // Mt[b][s] += wo
-    }
-    if (wi) {
-      k[4] = m_first_key(a.M, bM, r, sl[4]);   // M[b][r]  -= wi
-      k[5] = m_first_key(a.Mt, rT, b, sl[5]);  // Mt[r][b] -= wi
-      k[6] = m_first_key(a.M, bM, s, sl[6]);   // M[b][s]  += wi
-      k[7] = m_first_key(a.Mt, sT, b, sl[7]);  // Mt[s][b] += wi
-    }
-    if (wo) {
-      m_add_at<true>(a.M, r, rM, b, -wo, sl[0], k[0], flags);
-      m_add_at<true>(a.Mt, b, bT, r, -wo, sl[1], k[1], flags);
-      m_add_at<true>(a.M, s, sM, b, wo, sl[2], k[2], flags);
-      m_add_at<true>(a.Mt, b, bT, s, wo, sl[3], k[3], flags);
-    }
-    if (wi) {
-      m_add_at<true>(a.M, b, bM, r, -wi, sl[4], k[4], flags);
-      m_add_at<true>(a.Mt, r, rT, b, -wi, sl[5], k[5], flags);
-      m_add_at<true>(a.M, b, bM, s, wi, sl[6], k[6], flags);
-      m_add_at<true>(a.Mt, s, sT, b, wi, sl[7], k[7], flags);
-    }
+    if (wo == 0 && wi == 0) continue;
+    const Row bM = m_row(a.M, b);
+    int64_t sl[4];
+    int32_t k[4];
+    k[0] = m_first_key(a.M, rM, b, sl[0]);  // (M[r][b], M[b][r]) -= (wo, wi)
+    k[1] = m_first_key(a.M, sM, b, sl[1]);  // (M[s][b], M[b][s]) += (wo, wi)
+    k[2] = m_first_key(a.M, bM, r, sl[2]);  // (M[b][r], M[r][b]) -= (wi, wo)
+    k[3] = m_first_key(a.M, bM, s, sl[3]);  // (M[b][s], M[s][b]) += (wi, wo)
+    m_add_at<true>(a.M, r, rM, b, -wo, -wi, sl[0], k[0], flags);
+    m_add_at<true>(a.M, s, sM, b, wo, wi, sl[1], k[1], flags);
+    m_add_at<true>(a.M, b, bM, r, -wi, -wo, sl[2], k[2], flags);
+    m_add_at<true>(a.M, b, bM, s, wi, wo, sl[3], k[3], flags);
   }
   wo_sum = wave_sum(wo_sum);
   wi_sum = wave_sum(wi_sum);
@@ -1546,7 +1561,7 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
     atomicAdd(&a.bsize[s], 1);
     a.labels[n] = s;
   }
-  return 52 + 60 * kc + 32 + 32 + 4;
+  return 52 + 60 * kc + 32 + 32 + 4;  // per list entry: id, label, weight, 4 slots of 12 bytes
 }
 
 // The batch's sums over all workgroups' shares (nparts of them), by one
@@ -1651,17 +1666,16 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   }
 }
 
-// Rows queued by k_apply (key count past 1/4 of the slab, or an increment
-// parked), handled between two batches, zero counts dropped: a row whose live
-// keys and parked increments fit an eighth of its slab (and at most kCompact
-// slots) is compacted in place through LDS; any other
-// row moves to a slab >= 8*live+16 (direct once that reaches B/8) at the free
-// tail of the pool, and the row's parked increments are replayed into it.
-// Bucket sums are unchanged.  One launch serves M and M^T; its block 0 first
-// runs the batch's update step.
-constexpr int kCompact = 8192;
+// Rows queued by k_apply (key count past 1/slack of the slab, or an increment
+// parked), handled between two batches, dead keys (both counts 0) dropped: a
+// row whose live keys and parked increments fit 1/(2 slack) of its slab (and
+// at most kCompact slots) is compacted in place through LDS; any other row
+// moves to a slab >= 2*slack*live+16 (direct once that reaches B/8) at the
+// free tail of the pool, and the row's parked increments are replayed into it.  Bucket
+// sums are unchanged.  Block 0 first runs the batch's update step.
+constexpr int kCompact = 4096;
 constexpr int kGrowBlocks = 128;  // k_grow's grid
-constexpr int32_t kOvfCap = 1 << 22;  // parked increments per orientation
+constexpr int32_t kOvfCap = 1 << 22;  // parked increments
 
 struct GrowSide {
   int64_t* off;
@@ -1675,10 +1689,11 @@ struct GrowSide {
   int32_t ovf_cap;
   GrowState* gst;
   int64_t P;
+  int32_t slack;
 };
 
 struct GrowArgs {
-  GrowSide side[2];
+  GrowSide side;
   int32_t B;
   Sweep* sweep;
   const BlockPart* part;  // k_apply's per-workgroup shares of the batch
@@ -1688,17 +1703,18 @@ struct GrowArgs {
   int32_t tstride;
 };
 
-// Replays the parked increments of row a of one orientation into its slab
-// (o, c) (the only block touching row a in this launch); an increment that
-// still finds no slot asks the host for a rebuild.
+// Replays the parked increments of row a into its slab (o, c) (the only block
+// touching row a in this launch); an increment that still finds no slot asks
+// the host for a rebuild.
 __device__ void replay_row(const GrowSide& g, int32_t a, int64_t o, int32_t c, int32_t B,
                            Sweep* sw) {
   const int32_t no = min(g.gst->ovf_n, g.ovf_cap);
   for (int32_t i = threadIdx.x; i < no; i += blockDim.x) {
     if (g.ovf[i] != a) continue;
-    const int32_t b = g.ovf[g.ovf_cap + i], w = g.ovf[2 * g.ovf_cap + i];
+    const int32_t b = g.ovf[g.ovf_cap + i];
+    const int32_t dout = g.ovf[2 * g.ovf_cap + i], din = g.ovf[3 * g.ovf_cap + i];
     if (c == B) {
-      atomicAdd(&g.kv[o + b].v, w);
+      cnt_add(&g.kv[o + b], dout, din);
       continue;
     }
     const uint32_t mask = (uint32_t)c - 1;
@@ -1714,7 +1730,7 @@ __device__ void replay_row(const GrowSide& g, int32_t a, int64_t o, int32_t c, i
         }
       }
       if (k == b) {
-        atomicAdd(&g.kv[o + h].v, w);
+        cnt_add(&g.kv[o + h], dout, din);
         done = true;
       }
       h = (h + 1) & mask;
@@ -1728,10 +1744,12 @@ __device__ void replay_row(const GrowSide& g, int32_t a, int64_t o, int32_t c, i
 
 __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
   __shared__ int32_t s_k[kCompact];
-  __shared__ int32_t s_v[kCompact];
+  __shared__ int32_t s_o[kCompact];
+  __shared__ int32_t s_i[kCompact];
   __shared__ long long s_base;
   __shared__ int32_t s_cap, s_live;
   const int32_t B = ga.B;
+  const GrowSide& g = ga.side;
   if (ga.tstamp && threadIdx.x == 0) ga.tstamp[blockIdx.x] = wall_clock64();
   // block 0: the batch's sums and the sweep step, if k_apply ran this batch
   // (its per-workgroup shares were complete when this kernel started)
@@ -1743,48 +1761,48 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       if (threadIdx.x == 0) ga.sweep->applied = 0;
     }
   }
-  const int32_t n0 = ga.side[0].gst->n;
-  const int32_t n1 = ga.side[1].gst->n;
-  for (int32_t qi = blockIdx.x; qi < n0 + n1; qi += gridDim.x) {
-    const GrowSide& g = qi < n0 ? ga.side[0] : ga.side[1];
-    const int32_t a = g.gq[qi < n0 ? qi : qi - n0];
+  const int32_t nq = g.gst->n;
+  for (int32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const int32_t a = g.gq[qi];
     const int64_t o = g.off[a];
     const int32_t c = g.cap[a];
     const bool dold = c == B;
     if (threadIdx.x == 0) s_live = 0;
     __syncthreads();
     for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-      const int32_t k = dold ? j : g.kv[o + j].k;
-      if (k >= 0 && g.kv[o + j].v != 0) atomicAdd(&s_live, 1);
+      const Slot x = g.kv[o + j];
+      const int32_t k = dold ? j : x.k;
+      if (k >= 0 && (x.o != 0 || x.i != 0)) atomicAdd(&s_live, 1);
     }
     __syncthreads();
     const int32_t live = s_live;
     const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
     int64_t o2 = o;                   // the row's slab after this step
     int32_t c2 = c;
-    if (!dold && 8 * (live + pend) <= c && c <= kCompact) {
+    if (!dold && 2 * g.slack * (live + pend) <= c && c <= kCompact) {
       // in place: gather the live entries, clear, reinsert
       __syncthreads();
       if (threadIdx.x == 0) s_live = 0;
       __syncthreads();
       for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
         const Slot x = g.kv[o + j];
-        const int32_t k = x.k, v = x.v;
-        if (k >= 0 && v != 0) {
+        if (x.k >= 0 && (x.o != 0 || x.i != 0)) {
           const int32_t i = atomicAdd(&s_live, 1);
-          s_k[i] = k;
-          s_v[i] = v;
+          s_k[i] = x.k;
+          s_o[i] = x.o;
+          s_i[i] = x.i;
         }
       }
       __syncthreads();
-      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) g.kv[o + j] = Slot{-1, 0};
+      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) g.kv[o + j] = Slot{-1, 0, 0, 0};
       __threadfence_block();
       __syncthreads();
       const uint32_t mask = (uint32_t)c - 1;
       for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
         uint32_t h = hash32(s_k[i]) & mask;
         while (atomicCAS(&g.kv[o + h].k, -1, s_k[i]) != -1) h = (h + 1) & mask;
-        g.kv[o + h].v = s_v[i];
+        g.kv[o + h].o = s_o[i];
+        g.kv[o + h].i = s_i[i];
       }
       __threadfence_block();
       __syncthreads();
@@ -1796,7 +1814,7 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     } else {
       if (threadIdx.x == 0) {
         int64_t nc = 32;
-        while (nc < 8 * ((int64_t)live + pend) + 16) nc <<= 1;
+        while (nc < 2 * g.slack * ((int64_t)live + pend) + 16) nc <<= 1;
         if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
         if (8 * nc >= B) nc = B;
         const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
@@ -1818,23 +1836,26 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       const int32_t nc = s_cap;
       if (nc > 0) {
         const int64_t nb = s_base;
-        for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) g.kv[nb + i] = Slot{-1, 0};
+        for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) g.kv[nb + i] = Slot{-1, 0, 0, 0};
         __threadfence_block();
         __syncthreads();
         const bool dnew = nc == B;
         const uint32_t mask = (uint32_t)nc - 1;
         for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
           const Slot x = g.kv[o + j];
-          const int32_t k = x.k, v = x.v;
-          if (k < 0 || v == 0) continue;
-          g.kv[o + j].v = 0;  // the old slab stays in the pool as zeros until a rebuild
+          if (x.k < 0 || (x.o == 0 && x.i == 0)) continue;
+          // the old slab stays in the pool as zeros until a rebuild
+          g.kv[o + j].o = 0;
+          g.kv[o + j].i = 0;
           if (dnew) {
-            g.kv[nb + k].v = v;
+            g.kv[nb + x.k].o = x.o;
+            g.kv[nb + x.k].i = x.i;
             continue;
           }
-          uint32_t h = hash32(k) & mask;
-          while (atomicCAS(&g.kv[nb + h].k, -1, k) != -1) h = (h + 1) & mask;
-          g.kv[nb + h].v = v;
+          uint32_t h = hash32(x.k) & mask;
+          while (atomicCAS(&g.kv[nb + h].k, -1, x.k) != -1) h = (h + 1) & mask;
+          g.kv[nb + h].o = x.o;
+          g.kv[nb + h].i = x.i;
         }
         __threadfence_block();
         __syncthreads();
@@ -1878,11 +1899,12 @@ struct DevSpM {
   int32_t* gmark = nullptr;  // [N]
   int32_t* gq = nullptr;     // [N]
   int32_t* gpend = nullptr;  // [N]
-  int32_t* ovf = nullptr;    // [3][ovf_cap]
+  int32_t* ovf = nullptr;    // [4][ovf_cap]
   int32_t ovf_cap = 0;       // parked increments that fit
   GrowState* gst = nullptr;
   int64_t P = 0;      // allocated slots
   int64_t total = 0;  // slots in use after the last rebuild
+  int32_t slack = 2;  // usap_params.slab_slack
 };
 
 struct usap_engine {
@@ -1908,8 +1930,8 @@ struct usap_engine {
   int32_t* labels = nullptr;
   int32_t *d_out = nullptr, *d_in = nullptr, *ecnt_out = nullptr, *ecnt_in = nullptr,
           *bsize = nullptr;
-  DevSpM M, Mt;
-  int64_t *tmpM = nullptr, *tmpT = nullptr;
+  DevSpM M;  // block rows (both orientations of M)
+  int64_t* tmpM = nullptr;
   void* scan_tmp = nullptr;
   size_t scan_tmp_bytes = 0;
   // merge
@@ -2058,6 +2080,7 @@ MView view(const DevSpM& m, int32_t B) {
   v.gst = m.gst;
   v.B = B;
   v.bshift = bucket_shift(B);
+  v.slack = m.slack;
   return v;
 }
 
@@ -2118,8 +2141,7 @@ int rebuild_M(usap_engine* e) {
                                  e->ecnt_in, e->bsize);
   }
   k_caps<<<blocks_for((int64_t)B + 1, 256), 256, 0, st>>>(B, e->p.slab_slack, e->ecnt_out,
-                                                          e->ecnt_in, e->M.cap,
-                                                          e->Mt.cap, e->tmpM, e->tmpT);
+                                                          e->ecnt_in, e->M.cap, e->tmpM);
   HIPCHK(hipGetLastError());
   size_t bytes = 0;
   HIPCHK(rocprim::exclusive_scan(nullptr, bytes, e->tmpM, e->M.off, (int64_t)0, (size_t)B + 1,
@@ -2132,55 +2154,47 @@ int rebuild_M(usap_engine* e) {
   bytes = e->scan_tmp_bytes;
   HIPCHK(rocprim::exclusive_scan(e->scan_tmp, bytes, e->tmpM, e->M.off, (int64_t)0, (size_t)B + 1,
                                  rocprim::plus<int64_t>(), st));
-  bytes = e->scan_tmp_bytes;
-  HIPCHK(rocprim::exclusive_scan(e->scan_tmp, bytes, e->tmpT, e->Mt.off, (int64_t)0, (size_t)B + 1,
-                                 rocprim::plus<int64_t>(), st));
-  int64_t tot[2];
-  HIPCHK(hipMemcpyAsync(&tot[0], e->M.off + B, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&tot[1], e->Mt.off + B, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  int64_t tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, e->M.off + B, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  DevSpM* ms[2] = {&e->M, &e->Mt};
-  for (int i = 0; i < 2; ++i) {
-    DevSpM& m = *ms[i];
-    m.total = tot[i];
-    if (3 * m.total > m.P) {  // two thirds of the pool is headroom for rows that grow
-      dfree(e, m.kv);
-      const int64_t P = 3 * m.total + (int64_t)4 * B + 4096;
-      if (int rc = dalloc(e, &m.kv, (size_t)P)) return rc;
-      m.P = P;
-    }
-    if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
-        m.kv, m.total);
-    HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
-    HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
-    HIPCHK(hipMemsetAsync(m.gpend, 0, sizeof(int32_t) * B, st));
-    GrowState gs0{};
-    HIPCHK(hipMemcpyAsync(&gs0, m.gst, sizeof(GrowState), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    gs0.n = 0;
-    gs0.nospace = 0;
-    gs0.ovf_n = 0;
-    gs0.used = (unsigned long long)m.total;
-    HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    if (m.bs_rows < B) {
-      dfree(e, m.bs);
-      if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
-      m.bs_rows = B;
-    }
+  DevSpM& m = e->M;
+  m.total = tot;
+  m.slack = e->p.slab_slack;
+  if (3 * m.total > m.P) {  // two thirds of the pool is headroom for rows that grow
+    dfree(e, m.kv);
+    const int64_t P = 3 * m.total + (int64_t)4 * B + 4096;
+    if (int rc = dalloc(e, &m.kv, (size_t)P)) return rc;
+    m.P = P;
+  }
+  if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
+      m.kv, m.total);
+  HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
+  HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
+  HIPCHK(hipMemsetAsync(m.gpend, 0, sizeof(int32_t) * B, st));
+  GrowState gs0{};
+  HIPCHK(hipMemcpyAsync(&gs0, m.gst, sizeof(GrowState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  gs0.n = 0;
+  gs0.nospace = 0;
+  gs0.ovf_n = 0;
+  gs0.used = (unsigned long long)m.total;
+  HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (m.bs_rows < B) {
+    dfree(e, m.bs);
+    if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
+    m.bs_rows = B;
   }
   HIPCHK(hipMemsetAsync(e->build_flags, 0, sizeof(int32_t), st));
   if (e->E > 0) {
     if (B <= kSmallB)
       k_insert_small<<<256, 256, 0, st>>>(e->E, e->out_src, e->out_idx, e->out_w, e->labels,
-                                          view(e->M, B), view(e->Mt, B));
+                                          view(e->M, B));
     else
       k_insert_edges<<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
-          e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), view(e->Mt, B),
-          e->build_flags);
+          e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), e->build_flags);
   }
   k_bucket_sums<<<blocks_for(B, 4), 256, 0, st>>>(view(e->M, B));
-  k_bucket_sums<<<blocks_for(B, 4), 256, 0, st>>>(view(e->Mt, B));
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
   int32_t flags = 0;
@@ -2228,31 +2242,27 @@ std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log;
 const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
 int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
-// Growth counters of both orientations, read on the engine's own stream
-// (never the legacy null stream, which other engines' threads must not see).
-static bool read_grow_states(usap_engine* e, GrowState g[2]) {
-  DevSpM* ms[2] = {&e->M, &e->Mt};
-  for (int i = 0; i < 2; ++i)
-    if (hipMemcpyAsync(&g[i], ms[i]->gst, sizeof(GrowState), hipMemcpyDeviceToHost, e->stream) !=
-        hipSuccess)
-      return false;
+// Growth counters, read on the engine's own stream (never the legacy null
+// stream, which other engines' threads must not see).
+static bool read_grow_state(usap_engine* e, GrowState* g) {
+  if (hipMemcpyAsync(g, e->M.gst, sizeof(GrowState), hipMemcpyDeviceToHost, e->stream) !=
+      hipSuccess)
+    return false;
   return hipStreamSynchronize(e->stream) == hipSuccess;
 }
 void engine_grow_debug(usap_engine* e, long long out[4]) {
   out[0] = out[1] = out[2] = out[3] = 0;
-  GrowState g[2] = {};
-  if (!read_grow_states(e, g)) return;
-  for (int i = 0; i < 2; ++i) {
-    out[0] += (long long)g[i].grows;
-    out[1] += (long long)g[i].compactions;
-    out[2] += (long long)g[i].full;
-    out[3] += (long long)g[i].nospace_events;
-  }
+  GrowState g{};
+  if (!read_grow_state(e, &g)) return;
+  out[0] = (long long)g.grows;
+  out[1] = (long long)g.compactions;
+  out[2] = (long long)g.full;
+  out[3] = (long long)g.nospace_events;
 }
 int64_t engine_grows(usap_engine* e) {
-  GrowState g[2] = {};
-  if (!read_grow_states(e, g)) return 0;
-  return (int64_t)(g[0].grows + g[1].grows);
+  GrowState g{};
+  if (!read_grow_state(e, &g)) return 0;
+  return (int64_t)g.grows;
 }
 void engine_set_error(usap_engine* e, const char* msg) { e->err = msg; }
 }  // namespace usap
@@ -2274,7 +2284,7 @@ void usap_default_params(usap_params* p) {
   p->seed = 0;
   p->device = 0;
   p->verbose = 0;
-  p->slab_slack = 4;
+  p->slab_slack = 2;
   p->graphs = 0;
 }
 
@@ -2300,7 +2310,7 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->ecnt_out);
   dfree(e, e->ecnt_in);
   dfree(e, e->bsize);
-  for (DevSpM* m : {&e->M, &e->Mt}) {
+  for (DevSpM* m : {&e->M}) {
     dfree(e, m->off);
     dfree(e, m->cap);
     dfree(e, m->nkeys);
@@ -2313,7 +2323,6 @@ int usap_destroy(usap_engine* e) {
     dfree(e, m->gst);
   }
   dfree(e, e->tmpM);
-  dfree(e, e->tmpT);
   dfree(e, e->scan_tmp);
   dfree(e, e->best_s);
   dfree(e, e->best_hi);
@@ -2477,7 +2486,7 @@ int alloc_state(usap_engine* e) {
   TRY(dalloc(e, &e->ecnt_out, N));
   TRY(dalloc(e, &e->ecnt_in, N));
   TRY(dalloc(e, &e->bsize, N));
-  for (DevSpM* m : {&e->M, &e->Mt}) {
+  for (DevSpM* m : {&e->M}) {
     TRY(dalloc(e, &m->off, (size_t)N + 1));
     TRY(dalloc(e, &m->cap, N));
     TRY(dalloc(e, &m->nkeys, N));
@@ -2485,11 +2494,10 @@ int alloc_state(usap_engine* e) {
     TRY(dalloc(e, &m->gq, N));
     TRY(dalloc(e, &m->gpend, N));
     m->ovf_cap = (int32_t)std::min<int64_t>(kOvfCap, 2 * e->E + 4096);
-    TRY(dalloc(e, &m->ovf, (size_t)3 * m->ovf_cap));
+    TRY(dalloc(e, &m->ovf, (size_t)4 * m->ovf_cap));
     TRY(dalloc(e, &m->gst, 1));
   }
   TRY(dalloc(e, &e->tmpM, (size_t)N + 1));
-  TRY(dalloc(e, &e->tmpT, (size_t)N + 1));
   TRY(dalloc(e, &e->best_s, N));
   TRY(dalloc(e, &e->best_hi, N));
   TRY(dalloc(e, &e->best_lo, N));
@@ -2511,7 +2519,7 @@ int alloc_state(usap_engine* e) {
   hipStream_t st = e->stream;
   HIPCHK(hipMemsetAsync(e->status, 0, sizeof(Status), st));
   HIPCHK(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
-  for (DevSpM* m : {&e->M, &e->Mt}) HIPCHK(hipMemsetAsync(m->gst, 0, sizeof(GrowState), st));
+  HIPCHK(hipMemsetAsync(e->M.gst, 0, sizeof(GrowState), st));
   std::memset(e->h_sweep, 0, sizeof(Sweep));
   for (hipEvent_t& ev : e->ev) HIPCHK(hipEventCreate(&ev));
   return USAP_OK;
@@ -2572,12 +2580,11 @@ int usap_set_graph(usap_engine* e, const usap_graph* g) {
   HIPCHK(hipSetDevice(e->p.device));
   HIPCHK(hipStreamSynchronize(e->stream));
   const int64_t cap = std::min<int64_t>(kOvfCap, 2 * g->E + 4096);
-  for (DevSpM* m : {&e->M, &e->Mt})
-    if (cap > m->ovf_cap) {  // overflow buffers follow the edge count
-      dfree(e, m->ovf);
-      m->ovf_cap = (int32_t)cap;
-      if (int rc = dalloc(e, &m->ovf, (size_t)3 * m->ovf_cap)) return rc;
-    }
+  if (cap > e->M.ovf_cap) {  // the overflow buffer follows the edge count
+    dfree(e, e->M.ovf);
+    e->M.ovf_cap = (int32_t)cap;
+    if (int rc = dalloc(e, &e->M.ovf, (size_t)4 * e->M.ovf_cap)) return rc;
+  }
   if (int rc = load_graph(e, g, wtot)) return rc;
   e->merge_B = -1;
   return e->has_labels ? rebuild_M(e) : USAP_OK;
@@ -2644,7 +2651,7 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
     row.clear();
     for (int64_t j = 0; j < capv[a]; ++j) {
       const int32_t c = capv[a] == B ? (int32_t)j : kv[off[a] + j].k;
-      const int32_t v = kv[off[a] + j].v;
+      const int32_t v = kv[off[a] + j].o;  // M[a][c]
       if (c >= 0 && v != 0) row.emplace_back(c, v);
     }
     std::sort(row.begin(), row.end());
@@ -2690,7 +2697,6 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   const int32_t B = e->B;
   MergeArgs a;
   a.M = view(e->M, B);
-  a.Mt = view(e->Mt, B);
   a.d_out = e->d_out;
   a.d_in = e->d_in;
   a.lam = e->lam;
@@ -2871,7 +2877,6 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   NodalArgs a;
   a.g = gview(e);
   a.M = view(e->M, B);
-  a.Mt = view(e->Mt, B);
   a.labels = e->labels;
   a.d_out = e->d_out;
   a.d_in = e->d_in;
@@ -2903,7 +2908,6 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   ApplyArgs b;
   b.g = a.g;
   b.M = a.M;
-  b.Mt = a.Mt;
   b.labels = e->labels;
   b.d_out = e->d_out;
   b.d_in = e->d_in;
@@ -2925,20 +2929,21 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   b.tstride = G;
   const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
   GrowArgs ga;
-  DevSpM* ms[2] = {&e->M, &e->Mt};
-  for (int i = 0; i < 2; ++i) {
-    GrowSide& gs = ga.side[i];
-    gs.off = ms[i]->off;
-    gs.cap = ms[i]->cap;
-    gs.nkeys = ms[i]->nkeys;
-    gs.kv = ms[i]->kv;
-    gs.gmark = ms[i]->gmark;
-    gs.gq = ms[i]->gq;
-    gs.gpend = ms[i]->gpend;
-    gs.ovf = ms[i]->ovf;
-    gs.ovf_cap = ms[i]->ovf_cap;
-    gs.gst = ms[i]->gst;
-    gs.P = ms[i]->P;
+  {
+    const DevSpM& m = e->M;
+    GrowSide& gs = ga.side;
+    gs.off = m.off;
+    gs.cap = m.cap;
+    gs.nkeys = m.nkeys;
+    gs.kv = m.kv;
+    gs.gmark = m.gmark;
+    gs.gq = m.gq;
+    gs.gpend = m.gpend;
+    gs.ovf = m.ovf;
+    gs.ovf_cap = m.ovf_cap;
+    gs.gst = m.gst;
+    gs.P = m.P;
+    gs.slack = m.slack;
   }
   ga.B = B;
   ga.sweep = e->sweep;

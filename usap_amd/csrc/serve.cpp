@@ -229,8 +229,11 @@ int64_t usap_footprint(int32_t N, int64_t E) {
   const int64_t n = N, m = E;
   const int64_t graph = 16 * (n + 1) + 20 * m + 8 * (m + 1) + 8 * n + 96 * m;  // + lambda, large-node tables
   const int64_t state = 160 * n + 24 * std::min<int64_t>(1 << 22, 2 * m + 4096) + 65536;
-  const int64_t buckets = 2 * n * 512 * 4;                 // bucket sums at B = N
-  const int64_t pools = 2 * 8 * (2 * (8 * m + 64 * n) + 4 * n + 4096);  // M and M^T slabs
+  const int64_t buckets = n * 512 * 4;  // bucket sums at B = N
+  // block-row slabs: 16-byte slots, three times the rows' capacities (each a
+  // power of two below 2 * (2 * (out + in edges) + 16), slab_slack 2) plus
+  // headroom
+  const int64_t pools = 16 * (3 * (8 * m + 32 * n) + 4 * n + 4096);
   return graph + state + buckets + pools;
 }
 
