@@ -191,14 +191,14 @@ __device__ __forceinline__ int64_t m_slot(const MView& m, Row R, int32_t b) {
 // callers issue several probes before resolving any, so a wavefront has all
 // of its lookups in flight at once instead of one memory latency per lookup.
 struct Probe {
-  int64_t slot;
+  uint32_t h;  // slot within the row
   int32_t k, o, i;
 };
 
 __device__ __forceinline__ Probe m_probe(const MView& m, Row R, int32_t b) {
   Probe p;
-  p.slot = m_slot(m, R, b);
-  const Slot q = m.kv[p.slot];
+  p.h = R.c == m.B ? (uint32_t)b : hash32(b) & (uint32_t)(R.c - 1);
+  const Slot q = m.kv[R.o + p.h];
   p.k = q.k;
   p.o = q.o;
   p.i = q.i;
@@ -223,7 +223,7 @@ __device__ __forceinline__ Cnt m_value_more(const Slot* kv, int64_t o, int32_t c
 __device__ __forceinline__ Cnt m_value(const MView& m, Row R, int32_t b, const Probe& p) {
   if (R.c == m.B || p.k == b) return Cnt{p.o, p.i};
   if (p.k < 0) return Cnt{0, 0};
-  return m_value_more(m.kv, R.o, R.c, b, p.slot);
+  return m_value_more(m.kv, R.o, R.c, b, R.o + p.h);
 }
 
 // (M[a][b], M[b][a])
