@@ -30,11 +30,13 @@
 //        one wavefront per proposal
 //   k_nodal   : one nodal proposal per node against the frozen M
 //        (graph_parallel.hpp:990-1222), one wavefront per node
-//   k_apply   : accepted moves -> M, degrees (int32 atomics) and labels;
-//        equals the reference's full rebuild (graph_parallel.hpp:438-443)
+//   k_apply   : accepted moves -> M, degrees (atomics) and labels; equals
+//        the reference's full rebuild (graph_parallel.hpp:438-443); its
+//        block 0 runs the sweep's update step and convergence test
+//        (graph_parallel.hpp:433-458) from k_nodal's per-workgroup shares,
+//        so a whole sweep runs without the host
 //   k_grow    : row growth / compaction and overflow replay between batches;
-//        its block 0 runs the sweep's update step and convergence test
-//        (graph_parallel.hpp:433-458) so a whole sweep runs without the host
+//        acts on the batch's error and rebuild flags
 //   k_entropy : data_S over all non-zeros (graph_parallel.hpp:1281-1319)
 //
 // The nodal and apply kernels are latency bound (a few hundred bytes per node
@@ -93,7 +95,7 @@ struct Status {
 };
 
 // Device state of one nodal sweep (the reference's `update` task,
-// graph_parallel.hpp:433-458, run by block 0 of each batch's k_grow).
+// graph_parallel.hpp:433-458, run by block 0 of each batch's k_apply).
 struct Sweep {
   unsigned long long cum_lo, cum_hi;  // exact cumulative dS of the sweep (int128)
   unsigned long long stream0;         // random stream of the sweep's first batch
@@ -108,7 +110,7 @@ struct Sweep {
   int32_t stop;     // 0 run, 1 converged, 2 M needs a rebuild, 3 inconsistent M
   int32_t done;     // batches executed
   int32_t flags;    // OR of the batches' flags
-  int32_t applied;  // k_apply ran in the current batch (the update step is due)
+  int32_t stop_n;   // stop as the running batch's k_nodal saw it (0: the batch runs)
 };
 
 struct GraphView {
@@ -791,7 +793,7 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
 // k_grow writes its start at t[b] and its end at t[stride + b] with plain
 // stores: one atomic per workgroup on a shared word costs more than the
 // kernel's own tail.  Per batch slot: k_nodal's starts and ends (G each),
-// k_apply's (G each), k_grow's (kGrowBlocks each).
+// k_apply's (G + 1 each), k_grow's (kGrowBlocks each).
 __global__ void k_stamp_init(unsigned long long* t, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -804,8 +806,9 @@ __global__ __launch_bounds__(256) void k_stamp_reduce(const unsigned long long* 
                                                      int32_t grow_blocks, unsigned long long* out) {
   __shared__ unsigned long long s_lo[256], s_hi[256];
   const int32_t slot = blockIdx.x / 3, k = blockIdx.x % 3;
-  const int64_t base = (int64_t)slot * (4 * (int64_t)G + 2 * grow_blocks) + (int64_t)k * 2 * G;
-  const int32_t n = k < 2 ? G : grow_blocks;
+  const int64_t base = (int64_t)slot * (4 * (int64_t)G + 2 + 2 * grow_blocks) +
+                       (k == 0 ? 0 : k == 1 ? 2 * (int64_t)G : 4 * (int64_t)G + 2);
+  const int32_t n = k == 0 ? G : k == 1 ? G + 1 : grow_blocks;
   unsigned long long lo = ~0ull, hi = 0ull;
   for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
     const unsigned long long a = t[base + i], b = t[base + n + i];
@@ -1033,8 +1036,8 @@ __global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
 constexpr int kEdgeRegs = 4;     // list entries per lane per pass (256 per wavefront)
 constexpr int kScrNodal = 1024;  // k_nodal's LDS scratch per wavefront (one key bucket)
 
-// One workgroup's share of a batch, written by k_nodal / k_apply and summed
-// by block 0 of k_grow (no same-address atomics per node).
+// One workgroup's share of a batch, written by k_nodal and summed by block 0
+// of k_apply (no same-address atomics per node).
 struct BlockPart {
   long sum_hi;  // sum of the accepted finite dS of the block's nodes (int128)
   unsigned long sum_lo;
@@ -1043,6 +1046,11 @@ struct BlockPart {
   int32_t moves;
   int32_t ninf;
 };
+
+// Algorithmic bytes of one accepted move of a node with kc list entries: the
+// list (id, label, weight), per entry up to 4 slots of 12 bytes, the rows
+// and degrees of r and s, the label.
+__device__ __forceinline__ long apply_tally(int64_t kc) { return 52 + 60 * kc + 32 + 32 + 4; }
 
 struct NodalArgs {
   GraphView g;
@@ -1066,7 +1074,7 @@ struct NodalArgs {
   int32_t* prop_r;
   BlockPart* part;
   Status* status;
-  const Sweep* sweep;
+  Sweep* sweep;  // read, and stop_n written by block 0
   // kernel timers on (profiling): [0] earliest workgroup start, [1] latest
   // workgroup end of this launch, 100 MHz clock; else null
   unsigned long long* tstamp;  // workgroup stamps (start [b], end [tstride + b]) or null
@@ -1274,18 +1282,16 @@ __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, 
 // the proposal follows M, its rows' bucket sums and one bucket); s's rows and
 // degrees; then every lookup of the dS at once.
 template <bool kBig>
-__device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int32_t n, int32_t* tab,
-                          int32_t* scr) {
+__device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int32_t n, int32_t r,
+                                          int64_t o0, int64_t kco, int64_t i0, int64_t kci,
+                                          uint64_t stream0, int32_t* tab, int32_t* scr,
+                                          int32_t& acc_out, int32_t& inf_out, i128& dS_out) {
   const int lane = lane_id();
   const GraphView& g = a.g;
   const int32_t B = a.B;
-  const int32_t r = a.labels[n];
-  const int64_t o0 = g.out_ptr[n], i0 = g.in_ptr[n];
-  const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
   const int64_t kc = kco + kci;
   uint64_t x[4];
-  usap::philox4x64((uint64_t)n, 0, a.sweep->stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0,
-                   x);
+  usap::philox4x64((uint64_t)n, 0, stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0, x);
   const Row rM = m_row(a.M, r);
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   int64_t k_out = kco, k_in = kci;
@@ -1368,6 +1374,9 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
     a.prop_acc[item] = acc;
     a.prop_r[item] = r;
   }
+  acc_out = acc;
+  inf_out = inf ? 1 : 0;
+  dS_out = dS;
   return bytes;
 }
 
@@ -1376,27 +1385,67 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   __shared__ __align__(8) int32_t s_scr[kNodalWaves][kScrNodal];
   const int w = threadIdx.x / kWave;
   if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the previous k_grow consumed the queues
-    a.M.gst->n = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.M.gst->n = 0;  // the previous k_grow consumed the queues
     a.M.gst->ovf_n = 0;
+    // k_apply decides by this copy whether the batch runs: its block 0
+    // updates stop meanwhile
+    a.sweep->stop_n = a.sweep->stop;
   }
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
-  long bytes = 0;
-  if (n64 < a.end && !a.sweep->stop) {
+  long bytes = 0, abytes = 0;
+  int32_t acc = 0, inf = 0;
+  i128 dS = 0;
+  if (n64 < a.end) {
+    // the sweep's stop flag is read together with the node's first loads
     const int32_t n = (int32_t)n64;
-    const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
-    bytes = kc > kTabMaxDeg ? nodal_one<true>(a, item, n, s_tab[w], s_scr[w])
-                            : nodal_one<false>(a, item, n, s_tab[w], s_scr[w]);
+    const int32_t stop = a.sweep->stop;
+    const uint64_t stream0 = a.sweep->stream0;
+    const int32_t r = a.labels[n];
+    const int64_t o0 = a.g.out_ptr[n], i0 = a.g.in_ptr[n];
+    const int64_t kco = a.g.out_ptr[n + 1] - o0, kci = a.g.in_ptr[n + 1] - i0;
+    if (!stop) {
+      bytes = kco + kci > kTabMaxDeg
+                  ? nodal_one<true>(a, item, n, r, o0, kco, i0, kci, stream0, s_tab[w], s_scr[w],
+                                    acc, inf, dS)
+                  : nodal_one<false>(a, item, n, r, o0, kco, i0, kci, stream0, s_tab[w],
+                                     s_scr[w], acc, inf, dS);
+      if (acc) abytes = apply_tally(kco + kci);
+    }
   }
   __syncthreads();  // every wavefront is done with its scratch
-  unsigned long long* s_bytes = reinterpret_cast<unsigned long long*>(&s_scr[0][0]);
-  if (lane_id() == 0) s_bytes[w] = (unsigned long long)bytes;
+  // this block's share of the batch sums: accepted moves, the exact sum of
+  // their finite dS, the -inf ones, byte tallies
+  unsigned long long* s_p = reinterpret_cast<unsigned long long*>(&s_scr[0][0]);
+  if (lane_id() == 0) {
+    const u128 d = (acc && !inf) ? (u128)dS : (u128)0;
+    s_p[6 * w + 0] = (unsigned long long)bytes;
+    s_p[6 * w + 1] = (unsigned long long)abytes;
+    s_p[6 * w + 2] = (unsigned long long)d;
+    s_p[6 * w + 3] = (unsigned long long)(d >> 64);
+    s_p[6 * w + 4] = (unsigned long long)acc;
+    s_p[6 * w + 5] = (unsigned long long)(acc && inf);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int i = 0; i < kNodalWaves; ++i) t += s_bytes[i];
-    a.part[blockIdx.x].nodal_bytes = t;
+    unsigned long long nb = 0, ab = 0;
+    u128 sum = 0;
+    int32_t moves = 0, ninf = 0;
+    for (int i = 0; i < kNodalWaves; ++i) {
+      nb += s_p[6 * i + 0];
+      ab += s_p[6 * i + 1];
+      sum += ((u128)s_p[6 * i + 3] << 64) | (u128)s_p[6 * i + 2];
+      moves += (int32_t)s_p[6 * i + 4];
+      ninf += (int32_t)s_p[6 * i + 5];
+    }
+    BlockPart& P = a.part[blockIdx.x];
+    P.sum_lo = (unsigned long)sum;
+    P.sum_hi = (long)(sum >> 64);
+    P.nodal_bytes = nb;
+    P.apply_bytes = ab;
+    P.moves = moves;
+    P.ninf = ninf;
     if (a.tstamp) a.tstamp[a.tstride + blockIdx.x] = wall_clock64();
   }
 }
@@ -1422,7 +1471,8 @@ struct ApplyArgs {
   const int32_t* prop_inf;
   const int32_t* prop_acc;
   const int32_t* prop_r;
-  BlockPart* part;
+  const BlockPart* part;  // k_nodal's per-workgroup shares of the batch
+  uint32_t nparts;
   Status* status;
   Sweep* sweep;
   unsigned long long* tstamp;  // as NodalArgs::tstamp, for this kernel
@@ -1439,11 +1489,11 @@ __device__ double fx_to_double_dev(i128 v) {
 
 // graph_parallel.hpp:433-458 after the moves of one batch (sum: exact sum of
 // the batch's accepted finite dS, ninf: accepted -inf ones): cumulative dS of
-// the sweep, S += dS_sum, the moving-average convergence test; clears the
-// batch flags.  Same IEEE double operations as the host driver.
-__device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_t moves,
+// the sweep, S += dS_sum, the moving-average convergence test.  Same IEEE
+// double operations as the host driver.  The batch's error and rebuild flags
+// are k_grow's to act on (they may still be raised while this runs).
+__device__ void sweep_tail(Sweep* sw, u128 sum, int32_t ninf, int32_t moves,
                            unsigned long long nbytes, unsigned long long abytes) {
-  const int32_t flags = __hip_atomic_load(&st->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   sw->nodal_bytes += nbytes;
   sw->apply_bytes += abytes;
   const u128 cum = (((u128)sw->cum_hi << 64) | sw->cum_lo) + sum;
@@ -1455,17 +1505,7 @@ __device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_
   sw->S += dS_sum;
   sw->done += 1;
   sw->moves += (unsigned long long)moves;
-  // k_grow's other blocks may raise flags / ask for a rebuild meanwhile:
-  // convergence and errors override a rebuild request, as the step that
-  // decides comes after the batch
-  atomicOr(&sw->flags, flags);
-  if (flags & (kFlagMissing | kFlagNegative))
-    atomicExch(&sw->stop, 3);
-  else if (sw->check && -sw->mean < sw->thr * sw->S)
-    atomicExch(&sw->stop, 1);
-  else if (flags & kFlagFull)
-    atomicCAS(&sw->stop, 0, 2);
-  st->flags = 0;
+  if (sw->check && -sw->mean < sw->thr * sw->S) atomicExch(&sw->stop, 1);
 }
 
 // Moves accepted node n from block r to s: its edges' counts leave row and
@@ -1478,12 +1518,11 @@ __device__ void sweep_tail(Sweep* sw, Status* st, u128 sum, int32_t ninf, int32_
 // writing labels cannot race with their reads.  Returns the algorithmic bytes.
 template <bool kBig>
 __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t r, int32_t s,
+                                          int64_t o0, int64_t kco, int64_t i0, int64_t kci,
                                           int32_t* tab) {
   const int lane = lane_id();
   const GraphView& g = a.g;
   int32_t* flags = &a.status->flags;
-  const int64_t o0 = g.out_ptr[n], i0 = g.in_ptr[n];
-  const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
   const int64_t kc = kco + kci;
   const Row rM = m_row(a.M, r), sM = m_row(a.M, s);
   const Table T = tab_open<kBig>(tab, a.big_off, a.big_tab, n, kc);
@@ -1561,13 +1600,12 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
     atomicAdd(&a.bsize[s], 1);
     a.labels[n] = s;
   }
-  return 52 + 60 * kc + 32 + 32 + 4;  // per list entry: id, label, weight, 4 slots of 12 bytes
+  return apply_tally(kc);
 }
 
 // The batch's sums over all workgroups' shares (nparts of them), by one
 // block, then the sweep step.
-__device__ __forceinline__ void batch_tail(const BlockPart* part, uint32_t nparts, Status* st,
-                                           Sweep* sw) {
+__device__ __forceinline__ void batch_tail(const BlockPart* part, uint32_t nparts, Sweep* sw) {
   __shared__ unsigned long s_lo[kNodalWaves];
   __shared__ long s_hi[kNodalWaves];
   __shared__ unsigned long long s_nb[kNodalWaves], s_ab[kNodalWaves];
@@ -1608,61 +1646,47 @@ __device__ __forceinline__ void batch_tail(const BlockPart* part, uint32_t npart
       tmv += s_mv[i];
       tni += s_ni[i];
     }
-    sweep_tail(sw, st, tot, (int32_t)tni, (int32_t)tmv, tnb, tab);
+    sweep_tail(sw, tot, (int32_t)tni, (int32_t)tmv, tnb, tab);
   }
   __syncthreads();
 }
 
+// Block 0 runs the batch's update step from k_nodal's shares while blocks
+// 1.. apply the moves of items (block - 1) * kNodalWaves + wave.
 __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
-  __shared__ unsigned long long s_ab[kNodalWaves], s_lo[kNodalWaves];
-  __shared__ long s_hi[kNodalWaves];
-  __shared__ int32_t s_mv[kNodalWaves], s_ni[kNodalWaves];
-  if (a.sweep->stop) return;  // the same for every block of this launch
-  if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.sweep->applied = 1;
-  const int w = threadIdx.x / kWave;
-  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
-  const int64_t n64 = (int64_t)a.beg + item;
-  long bytes = 0;
-  int32_t mv = 0, ni = 0;
-  u128 dsum = 0;
-  if (n64 < a.end && a.prop_acc[item]) {
-    const int32_t n = (int32_t)n64;
-    mv = 1;
-    if (a.prop_inf[item])
-      ni = 1;
-    else
-      dsum = ((u128)(unsigned long)a.prop_hi[item] << 64) | (u128)a.prop_lo[item];
-    const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
-    bytes = kc > kTabMaxDeg ? apply_one<true>(a, n, a.prop_r[item], a.prop_s[item], s_tab[w])
-                            : apply_one<false>(a, n, a.prop_r[item], a.prop_s[item], s_tab[w]);
-  }
-  if (lane_id() == 0) {
-    s_ab[w] = (unsigned long long)bytes;
-    s_lo[w] = (unsigned long long)dsum;
-    s_hi[w] = (long)(dsum >> 64);
-    s_mv[w] = mv;
-    s_ni[w] = ni;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {  // this block's share of the batch sums, for k_grow's block 0
-    u128 sum = 0;
-    unsigned long long ab = 0;
-    int32_t moves = 0, ninf = 0;
-    for (int i = 0; i < kNodalWaves; ++i) {
-      ab += s_ab[i];
-      sum += ((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i];
-      moves += s_mv[i];
-      ninf += s_ni[i];
+  if (blockIdx.x == 0) {
+    if (a.tstamp && threadIdx.x == 0) a.tstamp[0] = wall_clock64();
+    if (a.sweep->stop_n == 0) batch_tail(a.part, a.nparts, a.sweep);
+    if (a.tstamp) {
+      __syncthreads();
+      if (threadIdx.x == 0) a.tstamp[a.tstride] = wall_clock64();
     }
-    BlockPart& P = a.part[blockIdx.x];
-    P.sum_lo = (unsigned long)sum;
-    P.sum_hi = (long)(sum >> 64);
-    P.apply_bytes = ab;
-    P.moves = moves;
-    P.ninf = ninf;
-    if (a.tstamp) a.tstamp[a.tstride + blockIdx.x] = wall_clock64();
+    return;
+  }
+  const int w = threadIdx.x / kWave;
+  const int64_t item = (int64_t)(blockIdx.x - 1) * kNodalWaves + w;
+  const int64_t n64 = (int64_t)a.beg + item;
+  const bool in = n64 < a.end;
+  const int32_t n = in ? (int32_t)n64 : a.beg;
+  // whether the batch runs, the proposal and the node's list bounds in one
+  // round of loads
+  const int32_t stop = a.sweep->stop_n;  // the same for every block of this launch
+  const int32_t acc = in ? a.prop_acc[item] : 0;
+  const int32_t pr = in ? a.prop_r[item] : 0, ps = in ? a.prop_s[item] : 0;
+  const int64_t o0 = a.g.out_ptr[n], i0 = a.g.in_ptr[n];
+  const int64_t kco = a.g.out_ptr[n + 1] - o0, kci = a.g.in_ptr[n + 1] - i0;
+  if (stop) return;
+  if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
+  if (in && acc) {
+    if (kco + kci > kTabMaxDeg)
+      apply_one<true>(a, n, pr, ps, o0, kco, i0, kci, s_tab[w]);
+    else
+      apply_one<false>(a, n, pr, ps, o0, kco, i0, kci, s_tab[w]);
+  }
+  if (a.tstamp) {
+    __syncthreads();
+    if (threadIdx.x == 0) a.tstamp[a.tstride + blockIdx.x] = wall_clock64();
   }
 }
 
@@ -1672,7 +1696,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
 // at most kCompact slots) is compacted in place through LDS; any other row
 // moves to a slab >= 2*slack*live+16 (direct once that reaches B/8) at the
 // free tail of the pool, and the row's parked increments are replayed into it.  Bucket
-// sums are unchanged.  Block 0 first runs the batch's update step.
+// sums are unchanged.  Block 0 also acts on the batch's flags.
 constexpr int kCompact = 4096;
 constexpr int kGrowBlocks = 128;  // k_grow's grid
 constexpr int32_t kOvfCap = 1 << 22;  // parked increments
@@ -1696,8 +1720,6 @@ struct GrowArgs {
   GrowSide side;
   int32_t B;
   Sweep* sweep;
-  const BlockPart* part;  // k_apply's per-workgroup shares of the batch
-  uint32_t nparts;
   Status* status;
   unsigned long long* tstamp;  // as NodalArgs::tstamp, for this kernel
   int32_t tstride;
@@ -1751,14 +1773,18 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
   const int32_t B = ga.B;
   const GrowSide& g = ga.side;
   if (ga.tstamp && threadIdx.x == 0) ga.tstamp[blockIdx.x] = wall_clock64();
-  // block 0: the batch's sums and the sweep step, if k_apply ran this batch
-  // (its per-workgroup shares were complete when this kernel started)
-  if (blockIdx.x == 0) {
-    const bool applied = ga.sweep->applied != 0;
-    __syncthreads();
-    if (applied) {
-      batch_tail(ga.part, ga.nparts, ga.status, ga.sweep);
-      if (threadIdx.x == 0) ga.sweep->applied = 0;
+  // the batch's flags (raised by its k_nodal and k_apply, both complete):
+  // errors override convergence, which overrides a rebuild request, as the
+  // decision comes after the batch
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int32_t f = ga.status->flags;
+    if (f) {
+      atomicOr(&ga.sweep->flags, f);
+      if (f & (kFlagMissing | kFlagNegative))
+        atomicExch(&ga.sweep->stop, 3);
+      else if (f & kFlagFull)
+        atomicCAS(&ga.sweep->stop, 0, 2);
+      ga.status->flags = 0;
     }
   }
   const int32_t nq = g.gst->n;
@@ -2833,7 +2859,7 @@ namespace {
 // these launches would add cache flushes between the kernels and slow them.
 int prof_begin(usap_engine* e, size_t nslots, int32_t batch) {
   const int32_t G = (int32_t)blocks_for(batch, kNodalWaves);
-  const size_t n = nslots * (4 * (size_t)G + 2 * kGrowBlocks) + 6 * nslots;
+  const size_t n = nslots * (4 * (size_t)G + 2 + 2 * kGrowBlocks) + 6 * nslots;
   if (e->tstamp_n < n) {
     dfree(e, e->tstamp);
     if (int rc = dalloc(e, &e->tstamp, n)) return rc;
@@ -2887,11 +2913,11 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   a.exact = e->p.ds_mode == USAP_DS_EXACT;
   a.seed = e->p.seed;
   a.bidx = bidx;
-  // profiled: this slot's workgroup stamps (k_nodal, k_apply: G starts + G
-  // ends each; k_grow: kGrowBlocks + kGrowBlocks)
+  // profiled: this slot's workgroup stamps (k_nodal: G starts + G ends;
+  // k_apply: G + 1 each; k_grow: kGrowBlocks each)
   const int32_t G = e->tstamp_G;
   unsigned long long* ts =
-      e->prof ? e->tstamp + ev_slot * (4 * (size_t)G + 2 * kGrowBlocks) : nullptr;
+      e->prof ? e->tstamp + ev_slot * (4 * (size_t)G + 2 + 2 * kGrowBlocks) : nullptr;
   a.tstamp = ts;
   a.tstride = G;
   a.big_off = e->big_off;
@@ -2922,12 +2948,13 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   b.prop_r = e->prop_r;
   b.big_off = e->big_off;
   b.big_tab = e->big_tab;
+  const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
   b.part = e->part;
+  b.nparts = grid;
   b.status = e->status;
   b.sweep = e->sweep;
   b.tstamp = ts ? ts + 2 * (size_t)G : nullptr;
-  b.tstride = G;
-  const unsigned grid = blocks_for((int64_t)end - beg, kNodalWaves);
+  b.tstride = G + 1;
   GrowArgs ga;
   {
     const DevSpM& m = e->M;
@@ -2947,15 +2974,13 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   }
   ga.B = B;
   ga.sweep = e->sweep;
-  ga.part = e->part;
-  ga.nparts = grid;
   ga.status = e->status;
-  ga.tstamp = ts ? ts + 4 * (size_t)G : nullptr;
+  ga.tstamp = ts ? ts + 4 * (size_t)G + 2 : nullptr;
   ga.tstride = kGrowBlocks;
   if (e->prof && (int64_t)grid > G)
     return set_err(e, USAP_ESTATE, "profiled batch larger than its stamp slot");
   k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
-  k_apply<<<grid, kWave * kNodalWaves, 0, st>>>(b);
+  k_apply<<<grid + 1, kWave * kNodalWaves, 0, st>>>(b);  // block 0: the update step
   k_grow<<<kGrowBlocks, 256, 0, st>>>(ga);
   HIPCHK(hipGetLastError());
   return USAP_OK;

@@ -27,7 +27,7 @@ def fx(hi, lo):
 
 
 class Engine:
-    """One graph resident on the device: labels, M (+ M^T), block degrees."""
+    """One graph resident on the device: labels, M (block rows holding both orientations), block degrees."""
 
     def __init__(self, graph, **params):
         self.graph = graph
