@@ -138,7 +138,7 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
   std::vector<int64_t> off(n + 1, 0);
   for (int32_t i = 0; i < n; ++i) off[i + 1] = off[i] + graphs[i].N;
   std::vector<int32_t> scratch((size_t)off[n]);
-  // double the worker count while graphs/s improves by at least 5%
+  // double the worker count while graphs/s improves by at least 2%
   double best = 0.0;
   int32_t best_w = 1, k = 0;
   for (int32_t w = 2; w <= max_workers; w *= 2) {
@@ -151,7 +151,7 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
     if (tried) tried[k] = w;
     if (rates) rates[k] = rate;
     ++k;
-    if (rate > 1.05 * best) {
+    if (rate > 1.02 * best) {
       best = rate;
       best_w = w;
     } else {

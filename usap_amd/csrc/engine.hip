@@ -2418,6 +2418,20 @@ int check_graph(const usap_graph* g, int64_t* wtot) {
   return USAP_OK;
 }
 
+// lambda[0..maxv] from one table shared by the process's engines (computed
+// again only when a graph needs more entries: batches of graphs would
+// otherwise spend most of their host time in log()).
+void lambda_prefix(int64_t maxv, int64_t* out) {
+  static std::mutex mu;
+  static std::vector<int64_t> tab;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int64_t)tab.size() <= maxv) {
+    tab.assign((size_t)maxv + 1, 0);
+    usap_lambda_table(maxv, tab.data());
+  }
+  std::memcpy(out, tab.data(), sizeof(int64_t) * ((size_t)maxv + 1));
+}
+
 #define TRY(x)          \
   do {                  \
     int _rc = (x);      \
@@ -2466,7 +2480,7 @@ int load_graph(usap_engine* e, const usap_graph* g, int64_t wtot) {
   }
   TRY(dalloc(e, &e->lam, wtot + 1));
   std::vector<int64_t> lam((size_t)wtot + 1);
-  usap_lambda_table(wtot, lam.data());
+  lambda_prefix(wtot, lam.data());
   std::vector<int32_t> src((size_t)E);
   for (int32_t x = 0; x < N; ++x)
     for (int64_t j = g->out_ptr[x]; j < g->out_ptr[x + 1]; ++j) src[j] = x;
