@@ -831,6 +831,15 @@ __global__ __launch_bounds__(256) void k_stamp_reduce(const unsigned long long* 
   }
 }
 
+// Growth state after a rebuild: nothing queued or parked, the pool used up to
+// the new slabs' end (the event counters are kept).
+__global__ void k_reset_grow(GrowState* g, unsigned long long used) {
+  g->n = 0;
+  g->nospace = 0;
+  g->ovf_n = 0;
+  g->used = used;
+}
+
 __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict__ map) {
   const int32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < N) labels[x] = map[labels[x]];
@@ -2197,15 +2206,7 @@ int rebuild_M(usap_engine* e) {
   HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
   HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
   HIPCHK(hipMemsetAsync(m.gpend, 0, sizeof(int32_t) * B, st));
-  GrowState gs0{};
-  HIPCHK(hipMemcpyAsync(&gs0, m.gst, sizeof(GrowState), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  gs0.n = 0;
-  gs0.nospace = 0;
-  gs0.ovf_n = 0;
-  gs0.used = (unsigned long long)m.total;
-  HIPCHK(hipMemcpyAsync(m.gst, &gs0, sizeof(GrowState), hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
+  k_reset_grow<<<1, 1, 0, st>>>(m.gst, (unsigned long long)m.total);
   if (m.bs_rows < B) {
     dfree(e, m.bs);
     if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
@@ -2765,7 +2766,7 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
     HIPCHK(hipStreamSynchronize(e->stream));
     if (best_dS)
       for (int32_t r = 0; r < B; ++r) best_dS[r] = fx_to_double(mk128(h[r], l[r]));
-  } else {
+  } else if (e->prof) {  // the kernel time is read below; else the caller's next step waits
     HIPCHK(hipStreamSynchronize(e->stream));
   }
   prof_add(e, USAP_K_MERGE, ev_ms(e, 0, 1), 1, 0.0);
