@@ -766,6 +766,46 @@ __global__ __launch_bounds__(256) void k_insert_small(int64_t E, const int32_t* 
   }
 }
 
+// M build for kSmallB < B <= kPairSortB (rows mostly direct, few distinct
+// block pairs, so per-edge atomics would pile onto the same counts): the
+// edges' block pairs a*B + b are sorted and summed per pair, then each pair
+// is added once.
+constexpr int32_t kPairSortB = 4096;
+
+__global__ void k_pair_keys(int64_t E, const int32_t* __restrict__ src,
+                            const int32_t* __restrict__ dst, const int32_t* __restrict__ w,
+                            const int32_t* __restrict__ labels, int32_t B, uint32_t* key,
+                            int32_t* val) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    key[e] = (uint32_t)labels[src[e]] * (uint32_t)B + (uint32_t)labels[dst[e]];
+    val[e] = w ? w[e] : 1;
+  }
+}
+
+__global__ void k_insert_pairs(int64_t E, const uint32_t* __restrict__ key,
+                               const int32_t* __restrict__ val, const int32_t* count, MView M,
+                               int32_t* flags) {
+  const int64_t n = min<int64_t>(*count, E);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t a = (int32_t)(key[e] / (uint32_t)M.B), b = (int32_t)(key[e] % (uint32_t)M.B);
+    const int32_t w = val[e];
+    if (a == b) {
+      m_add<false>(M, a, a, w, w, flags);
+    } else {
+      m_add<false>(M, a, b, w, 0, flags);
+      m_add<false>(M, b, a, 0, w, flags);
+    }
+  }
+}
+
+// Blocks holding other than one node (0 means every block is a single node).
+__global__ void k_count_nonsingle(int32_t B, const int32_t* __restrict__ bsize, int32_t* out) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < B; i += gridDim.x * blockDim.x)
+    if (bsize[i] != 1) atomicAdd(out, 1);
+}
+
 // Bucket sums (o + i) of every block row (one wavefront per row).
 __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
   __shared__ int32_t s_b[4][kNBK];
@@ -1459,6 +1499,49 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   }
 }
 
+// M build when every block holds one node (the identity start, B == N): block
+// row labels[n] is node n's own list -- out-edges as o, in-edges as i -- so
+// one wavefront builds it alone from its neighbour-block table, with no
+// contention from other wavefronts (graph_parallel.hpp:703-759).
+template <bool kBig>
+__device__ __forceinline__ void build_single_one(const GraphView& g, const int32_t* labels,
+                                                 const MView& M, const int64_t* big_off,
+                                                 int32_t* big_tab, int32_t n, int32_t* tab,
+                                                 int32_t* flags) {
+  const int lane = lane_id();
+  const int64_t o0 = g.out_ptr[n], i0 = g.in_ptr[n];
+  const int64_t kco = g.out_ptr[n + 1] - o0, kci = g.in_ptr[n + 1] - i0;
+  const int64_t kc = kco + kci;
+  const int32_t a = labels[n];
+  const Table T = tab_open<kBig>(tab, big_off, big_tab, n, kc);
+  const uint32_t mask = (uint32_t)T.cap - 1;
+  for (int64_t j = lane; j < kc; j += kWave) {
+    int32_t v, w;
+    list_entry(g, o0, kco, i0, kc, j, v, w);
+    tab_add(T.tk, j < kco ? T.two : T.twi, mask, labels[v], w);
+  }
+  wave_fence();
+  long x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+  const int32_t nent = tab_compact(T, -2, -2, x0, x1, x2, x3);
+  for (int32_t j = lane; j < nent; j += kWave) m_add<false>(M, a, T.tk[j], T.two[j], T.twi[j], flags);
+}
+
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_build_single(GraphView g,
+                                                                    const int32_t* labels, MView M,
+                                                                    const int64_t* big_off,
+                                                                    int32_t* big_tab,
+                                                                    int32_t* flags) {
+  __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
+  const int w = threadIdx.x / kWave;
+  const int64_t n = (int64_t)blockIdx.x * kNodalWaves + w;
+  if (n >= g.N) return;
+  const int64_t kc = (g.out_ptr[n + 1] - g.out_ptr[n]) + (g.in_ptr[n + 1] - g.in_ptr[n]);
+  if (kc > kTabMaxDeg)
+    build_single_one<true>(g, labels, M, big_off, big_tab, (int32_t)n, s_tab[w], flags);
+  else
+    build_single_one<false>(g, labels, M, big_off, big_tab, (int32_t)n, s_tab[w], flags);
+}
+
 // ---------------------------------------------------------------------------
 // Apply accepted moves: block rows of M, degrees, block sizes, batch sum
 // (graph_parallel.hpp:433-443; equal to the rebuild from the new labels)
@@ -2000,7 +2083,12 @@ struct usap_engine {
   uint64_t g_epoch = 0;
   uint64_t epoch = 0;  // bumped by every device allocation (pointers may change)
   int64_t graph_captures = 0;
-  int32_t* build_flags = nullptr;
+  int32_t* build_flags = nullptr;  // [0] build overflow flags, [1] blocks holding != 1 node, [2] distinct block pairs
+  uint32_t* pair_key[2] = {nullptr, nullptr};  // block-pair builds [E]
+  int32_t* pair_val[2] = {nullptr, nullptr};
+  int64_t pair_cap = 0;  // edges the pair buffers hold
+  void* pair_tmp = nullptr;
+  size_t pair_tmp_bytes = 0;
   int64_t* big_off = nullptr;
   int32_t* big_tab = nullptr;
   // entropy
@@ -2160,6 +2248,51 @@ void prof_add(usap_engine* e, int k, double ms, int64_t launches, double bytes) 
   e->kp[k].bytes += bytes;
 }
 
+// The block-pair build (k_pair_keys, sort, sum per pair, k_insert_pairs).
+int build_by_pairs(usap_engine* e) {
+  const int32_t B = e->B;
+  const int64_t E = e->E;
+  hipStream_t st = e->stream;
+  if (e->pair_cap < E) {  // first use, or the graph grew (usap_set_graph)
+    for (int i = 0; i < 2; ++i) {
+      dfree(e, e->pair_key[i]);
+      dfree(e, e->pair_val[i]);
+      if (int rc = dalloc(e, &e->pair_key[i], (size_t)E)) return rc;
+      if (int rc = dalloc(e, &e->pair_val[i], (size_t)E)) return rc;
+    }
+    e->pair_cap = E;
+  }
+  int bits = 1;
+  while (((uint64_t)1 << bits) < (uint64_t)B * (uint64_t)B) ++bits;
+  size_t b1 = 0, b2 = 0;
+  HIPCHK(rocprim::radix_sort_pairs(nullptr, b1, e->pair_key[0], e->pair_key[1], e->pair_val[0],
+                                   e->pair_val[1], (size_t)E, 0, bits, st));
+  HIPCHK(rocprim::reduce_by_key(nullptr, b2, e->pair_key[1], e->pair_val[1], (size_t)E,
+                                e->pair_key[0], e->pair_val[0], e->build_flags + 2,
+                                rocprim::plus<int32_t>(), rocprim::equal_to<uint32_t>(), st));
+  const size_t need = std::max(b1, b2);
+  if (need > e->pair_tmp_bytes) {
+    dfree(e, e->pair_tmp);
+    HIPCHK(hipMallocAsync(&e->pair_tmp, need, st));
+    e->pair_tmp_bytes = need;
+  }
+  const unsigned g = std::min<int64_t>(blocks_for(E, 256), 16384);
+  k_pair_keys<<<g, 256, 0, st>>>(E, e->out_src, e->out_idx, e->out_w, e->labels, B, e->pair_key[0],
+                                 e->pair_val[0]);
+  HIPCHK(hipGetLastError());
+  size_t bytes = e->pair_tmp_bytes;
+  HIPCHK(rocprim::radix_sort_pairs(e->pair_tmp, bytes, e->pair_key[0], e->pair_key[1], e->pair_val[0],
+                                   e->pair_val[1], (size_t)E, 0, bits, st));
+  bytes = e->pair_tmp_bytes;
+  HIPCHK(rocprim::reduce_by_key(e->pair_tmp, bytes, e->pair_key[1], e->pair_val[1], (size_t)E,
+                                e->pair_key[0], e->pair_val[0], e->build_flags + 2,
+                                rocprim::plus<int32_t>(), rocprim::equal_to<uint32_t>(), st));
+  k_insert_pairs<<<g, 256, 0, st>>>(E, e->pair_key[0], e->pair_val[0], e->build_flags + 2,
+                                    view(e->M, B), e->build_flags);
+  HIPCHK(hipGetLastError());
+  return USAP_OK;
+}
+
 // Full M build from the labels (graph_parallel.hpp:703-759).
 int rebuild_M(usap_engine* e) {
   const int32_t B = e->B;
@@ -2189,6 +2322,15 @@ int rebuild_M(usap_engine* e) {
   bytes = e->scan_tmp_bytes;
   HIPCHK(rocprim::exclusive_scan(e->scan_tmp, bytes, e->tmpM, e->M.off, (int64_t)0, (size_t)B + 1,
                                  rocprim::plus<int64_t>(), st));
+  // B == N: is every block a single node (rows built per node)?
+  const bool maybe_single = B == e->N && e->E > 0;
+  int32_t nonsingle = 1;
+  if (maybe_single) {
+    HIPCHK(hipMemsetAsync(e->build_flags + 1, 0, sizeof(int32_t), st));
+    k_count_nonsingle<<<std::min<unsigned>(blocks_for(B, 256), 1024u), 256, 0, st>>>(
+        B, e->bsize, e->build_flags + 1);
+    HIPCHK(hipMemcpyAsync(&nonsingle, e->build_flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  }
   int64_t tot = 0;
   HIPCHK(hipMemcpyAsync(&tot, e->M.off + B, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -2214,10 +2356,15 @@ int rebuild_M(usap_engine* e) {
   }
   HIPCHK(hipMemsetAsync(e->build_flags, 0, sizeof(int32_t), st));
   if (e->E > 0) {
-    if (B <= kSmallB)
+    if (maybe_single && nonsingle == 0)
+      k_build_single<<<blocks_for(e->N, kNodalWaves), kWave * kNodalWaves, 0, st>>>(
+          gview(e), e->labels, view(e->M, B), e->big_off, e->big_tab, e->build_flags);
+    else if (B <= kSmallB)
       k_insert_small<<<256, 256, 0, st>>>(e->E, e->out_src, e->out_idx, e->out_w, e->labels,
                                           view(e->M, B));
-    else
+    else if (B <= kPairSortB) {
+      if (int rc = build_by_pairs(e)) return rc;
+    } else
       k_insert_edges<<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
           e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), e->build_flags);
   }
@@ -2330,6 +2477,11 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->out_w);
   dfree(e, e->in_w);
   dfree(e, e->out_src);
+  for (int i = 0; i < 2; ++i) {
+    dfree(e, e->pair_key[i]);
+    dfree(e, e->pair_val[i]);
+  }
+  dfree(e, e->pair_tmp);
   dfree(e, e->lam);
   dfree(e, e->labels);
   dfree(e, e->d_out);
@@ -2552,7 +2704,7 @@ int alloc_state(usap_engine* e) {
   TRY(dalloc(e, &e->part, blocks_for(N, kNodalWaves)));
   TRY(dalloc(e, &e->sweep, 1));
   TRY(dalloc(e, &e->prop_r, N));
-  TRY(dalloc(e, &e->build_flags, 1));
+  TRY(dalloc(e, &e->build_flags, 3));
   TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
   e->h_sweep = (Sweep*)pin_get(sizeof(Sweep));
   e->h_ent_part = (long*)pin_get(sizeof(long) * 2 * kEntropyBlocks);
