@@ -74,6 +74,24 @@ def test_merge_propose_and_apply(usap, orc, graphs, gname, init):
         _assert_same_state(eng, o)
 
 
+@pytest.mark.parametrize("K", [1, 3])
+def test_merge_with_few_proposals(usap, orc, graphs, K):
+    """K below the workgroup's four wavefronts: k_merge runs K wavefronts,
+    which order row r's keys in LDS for the whole workgroup."""
+    g = graphs["c1"]
+    lab, B = usap.scc_init(g)
+    eng, o = _engine_and_oracle(usap, orc, g, lab, B, seed=13, K=K)
+    for stream in range(2):
+        s, d = eng.merge_propose(stream)
+        os_, od = o.block_merge(stream)
+        np.testing.assert_array_equal(s, os_)
+        assert d == od
+        btm = eng.B // 2
+        eng.merge_apply(btm)
+        assert o.carry_out(btm) == 0
+        _assert_same_state(eng, o)
+
+
 @pytest.mark.parametrize("gname", ["c1", "weighted_selfloop_hubs"])
 @pytest.mark.parametrize("init", ["scc", "identity"])
 @pytest.mark.parametrize("mode", [0, 1])

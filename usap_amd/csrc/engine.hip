@@ -985,8 +985,80 @@ __device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r
   return acc;
 }
 
+// Row r of a k_merge workgroup in LDS, shared by its K proposals: its
+// non-zero keys in ascending order with both counts and the running sum of
+// o + i (n = -1: the row has more than kRowLds non-zero keys and the
+// proposals read it from M).
+constexpr int kRowLds = 256;
+struct RowLds {
+  const int32_t* k;
+  const int32_t* o;
+  const int32_t* i;
+  const long* cum;
+  int32_t n;
+};
+
+// Index of the first entry whose running sum exceeds t (t < the row's total).
+__device__ __forceinline__ int32_t rowlds_pick(const RowLds& L, int64_t t) {
+  int32_t lo = 0, hi = L.n - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (L.cum[mid] > t)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return L.k[lo];
+}
+
+// Position of key b among the row's keys, or -1.
+__device__ __forceinline__ int32_t rowlds_find(const RowLds& L, int32_t b) {
+  int32_t lo = 0, hi = L.n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (L.k[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < L.n && L.k[lo] == b ? lo : -1;
+}
+
+// merge_overlap_lane with row r from LDS: r's entries probe row s, or, when
+// s's slab is at most four times r's key count, s's slab is scanned and its
+// keys are looked up in r.
+__device__ i128 merge_overlap_lds(const MView& m, const int64_t* lam, int32_t r, int32_t s,
+                                  const RowLds& L) {
+  const int lane = lane_id();
+  const Row RS = m_row(m, s);
+  i128 acc = 0;
+  if (RS.c <= 4 * L.n) {
+    const bool direct = RS.c == m.B;
+    for (int32_t j = lane; j < RS.c; j += kWave) {
+      const Slot q = m.kv[RS.o + j];
+      const int32_t i = direct ? j : q.k;
+      if (i < 0 || (q.o <= 0 && q.i <= 0) || i == r || i == s) continue;
+      const int32_t p = rowlds_find(L, i);
+      if (p < 0) continue;
+      const int32_t o1 = L.o[p], i1 = L.i[p];
+      if (o1 > 0 && q.o > 0) acc += Lterm(lam, o1) + Lterm(lam, q.o) - Lterm(lam, (int64_t)o1 + q.o);
+      if (i1 > 0 && q.i > 0) acc += Lterm(lam, i1) + Lterm(lam, q.i) - Lterm(lam, (int64_t)i1 + q.i);
+    }
+  } else {
+    for (int32_t e = lane; e < L.n; e += kWave) {
+      const int32_t i = L.k[e];
+      if (i == r || i == s) continue;
+      const Cnt c2 = m_value(m, RS, i, m_probe(m, RS, i));
+      const int32_t o1 = L.o[e], i1 = L.i[e];
+      if (o1 > 0 && c2.o > 0) acc += Lterm(lam, o1) + Lterm(lam, c2.o) - Lterm(lam, (int64_t)o1 + c2.o);
+      if (i1 > 0 && c2.i > 0) acc += Lterm(lam, i1) + Lterm(lam, c2.i) - Lterm(lam, (int64_t)i1 + c2.i);
+    }
+  }
+  return acc;
+}
+
 __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* scr,
-                          int32_t& s_out, i128& dS_out) {
+                          const RowLds& L, int32_t& s_out, i128& dS_out) {
   const int32_t B = a.B;
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   const int64_t k = dor + dir;
@@ -997,7 +1069,8 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
     s = (int32_t)usap::idx_of(x[2], B);
   } else {
     long bytes = 0;
-    const int32_t u = select_combined(a.M, r, usap::idx_of(x[0], k), -1, scr, kScr, bytes);
+    const int64_t t0 = (int64_t)usap::idx_of(x[0], k);
+    const int32_t u = L.n >= 0 ? rowlds_pick(L, t0) : select_combined(a.M, r, t0, -1, scr, kScr, bytes);
     const int64_t da = (int64_t)a.d_out[u] + a.d_in[u];
     if (usap::uniform_branch(x[1], da, B)) {
       s = usap::uniform_not_r(x[2], r, B);
@@ -1019,7 +1092,7 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
   const int64_t Msr = crs.i, Mss = m_get(a.M, s, s).o;
   const int64_t dos = a.d_out[s], dis = a.d_in[s];
   const int64_t dos2 = dos + dor, dis2 = dis + dir;
-  i128 acc = merge_overlap_lane(a.M, lam, r, s);
+  i128 acc = L.n >= 0 ? merge_overlap_lds(a.M, lam, r, s, L) : merge_overlap_lane(a.M, lam, r, s);
   acc = wave_sum128(acc);
   bool inf = false;
   const i128 old_c = Tterm(lam, Mrr, dir, dor, inf) + Tterm(lam, Mrs, dis, dor, inf) +
@@ -1039,15 +1112,74 @@ __global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
   __shared__ int32_t s_idx[kMergeWaves];
   __shared__ int32_t s_s[kMergeWaves];
   __shared__ int32_t s_scr[kMergeWaves][kScr];
+  __shared__ int32_t s_rk[kRowLds], s_ro[kRowLds], s_ri[kRowLds];
+  __shared__ long s_cum[kRowLds];
+  __shared__ int32_t s_rn;
   const int32_t r = blockIdx.x;
   const int w = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
+  // row r's non-zero keys into LDS once for the K proposals (unordered), then
+  // ordered by key (ranks: keys are distinct), then the running sums
+  const Row R = m_row(a.M, r);
+  if (threadIdx.x == 0) s_rn = R.c <= 8 * kRowLds ? 0 : kRowLds + 1;
+  __syncthreads();
+  if (s_rn == 0) {
+    const bool direct = R.c == a.B;
+    for (int32_t j = threadIdx.x; j < R.c; j += blockDim.x) {
+      const Slot q = a.M.kv[R.o + j];
+      const int32_t key = direct ? j : q.k;
+      if (key >= 0 && (q.o > 0 || q.i > 0)) {
+        const int32_t p = atomicAdd(&s_rn, 1);
+        if (p < kRowLds) {
+          s_rk[p] = key;
+          s_ro[p] = q.o;
+          s_ri[p] = q.i;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int32_t rn = s_rn;
+  RowLds L{s_rk, s_ro, s_ri, s_cum, rn <= kRowLds ? rn : -1};
+  if (L.n > 0) {
+    constexpr int kPer = kRowLds / kWave;  // entries per thread (blockDim >= one wavefront)
+    int32_t mk[kPer], mo[kPer], mi[kPer], rank[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int32_t e = threadIdx.x + q * blockDim.x;
+      rank[q] = -1;
+      if (e < L.n) {
+        mk[q] = s_rk[e];
+        mo[q] = s_ro[e];
+        mi[q] = s_ri[e];
+        rank[q] = 0;
+        for (int32_t j = 0; j < L.n; ++j) rank[q] += s_rk[j] < mk[q];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      if (rank[q] < 0) continue;
+      s_rk[rank[q]] = mk[q];
+      s_ro[rank[q]] = mo[q];
+      s_ri[rank[q]] = mi[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long c = 0;
+      for (int32_t j = 0; j < L.n; ++j) {
+        c += (long)s_ro[j] + s_ri[j];
+        s_cum[j] = c;
+      }
+    }
+    __syncthreads();
+  }
   int32_t bidx = -1, bs = -1;
   i128 bd = 0;
   for (int32_t idx = w; idx < a.K; idx += nw) {
     int32_t s;
     i128 d;
-    merge_one(a, r, idx, s_scr[w], s, d);
+    merge_one(a, r, idx, s_scr[w], L, s, d);
     if (bidx < 0 || d < bd) {  // strict less: the earliest proposal wins ties
       bidx = idx;
       bs = s;
