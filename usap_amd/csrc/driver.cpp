@@ -72,9 +72,12 @@ double ms_since(clk::time_point t0) {
 
 // The nodal phase at a fixed B (graph_parallel.hpp:381-478): sweeps of
 // batches until the moving-average convergence test fires (threshold thr) or
-// max_nodal_itr sweeps; the entropy is recomputed before each sweep
-// (prepare_batch) and after the last.  Advances nodal_stream by the batches
-// run; S receives the final MDL.
+// max_nodal_itr sweeps.  The reference recomputes the entropy before each
+// sweep (prepare_batch); S only enters the convergence test, so it is
+// computed for the sweeps that run the test (itr >= avg_window - 1) and the
+// sweeps before run with S = 0 -- the moves, the dS sums and the stop
+// decisions are the same.  Advances nodal_stream by the batches run; S
+// receives the last sweep's S (callers recompute the MDL after the phase).
 int nodal_phase(usap_engine* e, const usap_params& p, int32_t batch, int32_t B, double thr,
                 uint64_t& nodal_stream, std::vector<double>& itr_dS, usap_stats& st, double& S) {
   const int32_t W = p.avg_window;
@@ -84,7 +87,9 @@ int nodal_phase(usap_engine* e, const usap_params& p, int32_t batch, int32_t B, 
   for (int32_t itr = 0; itr < p.max_nodal_itr; ++itr) {
     // prepare_batch
     auto t0 = clk::now();
-    if ((rc = usap_entropy(e, &S, nullptr, nullptr))) return rc;
+    S = 0.0;
+    if (itr >= W - 1)
+      if ((rc = usap_entropy(e, &S, nullptr, nullptr))) return rc;
     st.ms_entropy += ms_since(t0);
     double mean = 0;
     if (itr >= W - 1) {

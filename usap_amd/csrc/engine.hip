@@ -1121,9 +1121,10 @@ __global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
   // row r's non-zero keys into LDS once for the K proposals (unordered), then
   // ordered by key (ranks: keys are distinct), then the running sums
   const Row R = m_row(a.M, r);
-  if (threadIdx.x == 0) s_rn = R.c <= 8 * kRowLds ? 0 : kRowLds + 1;
+  const bool fits = R.c <= 8 * kRowLds;  // the same for every thread
+  if (threadIdx.x == 0) s_rn = 0;
   __syncthreads();
-  if (s_rn == 0) {
+  if (fits) {
     const bool direct = R.c == a.B;
     for (int32_t j = threadIdx.x; j < R.c; j += blockDim.x) {
       const Slot q = a.M.kv[R.o + j];
@@ -1139,7 +1140,7 @@ __global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
     }
   }
   __syncthreads();
-  const int32_t rn = s_rn;
+  const int32_t rn = fits ? s_rn : kRowLds + 1;
   RowLds L{s_rk, s_ro, s_ri, s_cum, rn <= kRowLds ? rn : -1};
   if (L.n > 0) {
     constexpr int kPer = kRowLds / kWave;  // entries per thread (blockDim >= one wavefront)
