@@ -189,6 +189,8 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     const auto t_phase = clk::now();
     auto t0 = clk::now();
     if ((rc = usap_merge_propose(e, p.K, merge_stream++, nullptr, nullptr))) break;
+    // wait for the proposals here so that ms_merge holds the merge kernel
+    if ((rc = usap_merge_result_fx(e, nullptr, nullptr, nullptr))) break;
     st.merge_proposals += (int64_t)B * p.K;
     st.ms_merge += ms_since(t0);
     t0 = clk::now();
