@@ -1923,7 +1923,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
 // free tail of the pool, and the row's parked increments are replayed into it.  Bucket
 // sums are unchanged.  Block 0 also acts on the batch's flags.
 constexpr int kCompact = 4096;
-constexpr int kGrowBlocks = 128;  // k_grow's grid
+constexpr int kGrowBlocks = 512;  // k_grow's grid
 constexpr int32_t kOvfCap = 1 << 22;  // parked increments
 
 struct GrowSide {
