@@ -1,6 +1,6 @@
 """CPU tests of the product's host side: the C-ABI library loads and exports
-every declared entry point, graph ingest (TSV, CSR/CSC order), the DCSBM
-generator and the quality metrics."""
+every declared entry point, graph ingest (TSV, CSR/CSC order), the SCC
+initial partition, the DCSBM generator and the quality metrics."""
 import ctypes as C
 import os
 import re
@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import ROOT
+from conftest import ROOT, random_graph
 
 
 def test_abi_exports_every_declared_symbol(usap):
@@ -106,3 +106,36 @@ def test_metrics_against_sklearn(usap):
     assert usap.pairwise_precision_recall(t, t)[:2] == (1.0, 1.0)
     prec, rec, _, _ = usap.pairwise_precision_recall(t, np.array([0, 0, 0, 0]))
     assert rec == 1.0 and abs(prec - 2 / 6) < 1e-12
+
+
+def _by_source(g):
+    """The same edges listed by source (the file order of a sorted edge list):
+    every in-list then names its sources in ascending order."""
+    import usap_amd
+    src = np.repeat(np.arange(g.N, dtype=np.int32), np.diff(g.out_ptr))
+    return usap_amd.Graph.from_edges(src, g.out_idx, g.N)
+
+
+@pytest.mark.parametrize("kind", ["shuffled", "by_source", "generator", "chain"])
+@pytest.mark.parametrize("depth", [1, 20, 300])
+def test_scc_init_matches_oracle(usap, orc, kind, depth):
+    """usap_scc_init (a cursor per stacked node instead of one stack entry per
+    edge; the in-lists taken as the transpose when they are source-ascending,
+    else a counting sort) against the oracle's literal restatement of
+    _scc/_dfs_itr (graph_parallel.hpp:514-603): same labels and B, with
+    multi-edges, self-loops, hubs, isolated nodes and a DFS as deep as N."""
+    if kind == "shuffled":
+        g = random_graph(2500, 15000, seed=9, self_loops=True, hubs=2, isolated=6)
+    elif kind == "by_source":
+        g = _by_source(random_graph(2500, 15000, seed=9, self_loops=True, hubs=2, isolated=6))
+    elif kind == "generator":
+        g = usap.config_graph("c1")
+    else:  # one long path with a few back edges: the first DFS goes N deep
+        n = 30000
+        src = np.concatenate([np.arange(n - 1), [n - 1, 17000, 900]])
+        dst = np.concatenate([np.arange(1, n), [0, 5, 29000]])
+        g = usap.Graph.from_edges(src, dst, n)
+    lab, B = usap.scc_init(g, depth)
+    olab, oB = orc.Oracle(g).scc(depth)
+    assert B == oB
+    np.testing.assert_array_equal(lab, olab)

@@ -82,38 +82,74 @@ int usap_scc_init(const usap_graph* g, int32_t dfs_depth, int32_t* labels, int32
   if (!g || !labels || !B || dfs_depth < 1) return USAP_EINVAL;
   const int32_t N = g->N;
   const int64_t E = g->E;
+  // The reference's first DFS pushes every unvisited out-neighbour of the
+  // node it expands, explores the last one pushed first, skips entries found
+  // visited when they come up, and finishes a node when it is on top again.
+  // A cursor per stacked node that walks its out-list backwards and descends
+  // into the first neighbour still unvisited gives the same expansion and
+  // finish order (a neighbour's colour only grows between the push and the
+  // pop) with at most N frames instead of one entry per edge.
   std::vector<uint8_t> color((size_t)N, 0);
   std::vector<int32_t> finish;
   finish.reserve((size_t)N);
-  std::vector<int32_t> stk;
-  stk.reserve((size_t)N + 16);
+  std::vector<int32_t> fv((size_t)N);
+  std::vector<int64_t> fe((size_t)N);
   for (int32_t root = 0; root < N; ++root) {
     if (color[root]) continue;
-    stk.push_back(root);
-    while (!stk.empty()) {
-      const int32_t v = stk.back();
-      if (color[v]) {
-        stk.pop_back();
-        if (color[v] == 1) {
-          finish.push_back(v);
-          color[v] = 2;
+    int32_t top = 0;
+    fv[0] = root;
+    fe[0] = g->out_ptr[root + 1];
+    color[root] = 1;
+    while (top >= 0) {
+      const int32_t v = fv[top];
+      const int64_t lo = g->out_ptr[v];
+      int64_t e = fe[top];
+      int32_t next = -1;
+      while (e > lo) {
+        const int32_t w = g->out_idx[--e];
+        if (!color[w]) {
+          next = w;
+          break;
         }
+      }
+      fe[top] = e;
+      if (next >= 0) {
+        color[next] = 1;
+        ++top;
+        fv[top] = next;
+        fe[top] = g->out_ptr[next + 1];
       } else {
-        color[v] = 1;
-        for (int64_t e = g->out_ptr[v]; e < g->out_ptr[v + 1]; ++e)
-          if (!color[g->out_idx[e]]) stk.push_back(g->out_idx[e]);
+        finish.push_back(v);
+        color[v] = 2;
+        --top;
       }
     }
   }
-  // transpose listed by source ascending (graph_parallel.hpp:517-522)
+  // The transpose listed by source ascending (graph_parallel.hpp:517-522).
+  // The in-lists are that transpose whenever each lists its sources in
+  // ascending order (edges given by source, as loaded files and the
+  // generator give them); otherwise it is rebuilt by a counting sort.
   std::vector<int64_t> tp((size_t)N + 1, 0);
-  std::vector<int32_t> ti((size_t)E);
   for (int64_t e = 0; e < E; ++e) tp[g->out_idx[e] + 1]++;
+  bool use_in = g->in_ptr && g->in_idx && g->in_ptr[0] == 0;
+  for (int32_t v = 0; v < N && use_in; ++v) {
+    const int64_t a = g->in_ptr[v], b = g->in_ptr[v + 1];
+    use_in = b - a == tp[v + 1];
+    for (int64_t e = a + 1; e < b && use_in; ++e) use_in = g->in_idx[e - 1] <= g->in_idx[e];
+  }
   for (int32_t v = 0; v < N; ++v) tp[v + 1] += tp[v];
-  std::vector<int64_t> fill(tp.begin(), tp.end() - 1);
-  for (int32_t i = 0; i < N; ++i)
-    for (int64_t e = g->out_ptr[i]; e < g->out_ptr[i + 1]; ++e) ti[fill[g->out_idx[e]]++] = i;
+  std::vector<int32_t> built;
+  const int32_t* ti = g->in_idx;
+  if (!use_in) {
+    built.resize((size_t)E);
+    std::vector<int64_t> fill(tp.begin(), tp.end() - 1);
+    for (int32_t i = 0; i < N; ++i)
+      for (int64_t e = g->out_ptr[i]; e < g->out_ptr[i + 1]; ++e) built[fill[g->out_idx[e]]++] = i;
+    ti = built.data();
+  }
   std::vector<uint8_t> visited((size_t)N, 0);
+  std::vector<int32_t> stk;
+  stk.reserve((size_t)N + 16);
   int32_t b = 0;
   for (auto it = finish.rbegin(); it != finish.rend(); ++it) {
     if (visited[*it]) continue;
