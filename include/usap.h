@@ -359,7 +359,10 @@ int usap_golden_update(int64_t* slot_B, double* slot_S, int64_t* slot_id, int64_
 
 /* ---- host utilities (no device needed) ---------------------------------- */
 
-/* SCC initial partition.  ≙ _scc + _dfs_itr (graph_parallel.hpp:514-603). */
+/* SCC initial partition.  ≙ _scc + _dfs_itr (graph_parallel.hpp:514-603).
+ * Reads the out-lists; the in-lists (optional, NULL allowed) are used as the
+ * transpose when each lists its sources in ascending order with the
+ * out-lists' in-degrees, else the transpose is built from the out-lists. */
 int usap_scc_init(const usap_graph* g, int32_t dfs_depth, int32_t* labels, int32_t* B);
 
 /* Model entropy term.  ≙ graph_parallel.hpp:1314-1316 in double. */
