@@ -139,3 +139,19 @@ def test_scc_init_matches_oracle(usap, orc, kind, depth):
     olab, oB = orc.Oracle(g).scc(depth)
     assert B == oB
     np.testing.assert_array_equal(lab, olab)
+
+
+def test_scc_init_without_in_lists(usap, orc):
+    """The in-lists are optional for usap_scc_init: without them the transpose
+    is built from the out-lists, with the same result."""
+    from usap_amd import _lib
+    g = _by_source(random_graph(1500, 9000, seed=4, self_loops=True, hubs=1))
+    gs = g.as_struct()
+    gs.in_ptr = None
+    gs.in_idx = None
+    lab = np.zeros(g.N, np.int32)
+    B = C.c_int32()
+    assert _lib.lib().usap_scc_init(C.byref(gs), 20, lab, C.byref(B)) == 0
+    olab, oB = orc.Oracle(g).scc(20)
+    assert B.value == oB
+    np.testing.assert_array_equal(lab, olab)
