@@ -79,7 +79,7 @@ constexpr int kNBK = 512;                // bucket sums per block row
 constexpr int kScr = 3 * kTab;           // per-wavefront LDS scratch (ints) for one bucket
 constexpr int kDegLds = 1024;            // k_degrees privatises its counters for B <= this
 constexpr int kSmallB = 128;             // k_insert_small: B*B counts in LDS
-constexpr int kEntropyBlocks = 1024;
+constexpr int kEntropyBlocks = 4096;  // k_entropy grid at most
 constexpr int kNumSnap = 4;
 
 enum : int32_t {
@@ -2997,17 +2997,21 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
 int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
   if (int rc = check_engine(e)) return rc;
   hipStream_t st = e->stream;
+  // about four pool slots per thread, at most kEntropyBlocks workgroups (a
+  // scan of 1 GB pools at B = 250k is bound by the loads in flight)
+  const int64_t work = std::max<int64_t>(e->M.P, e->B);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(kEntropyBlocks, blocks_for(work, 1024)));
   ev_rec(e, 0);
-  k_entropy<<<kEntropyBlocks, 256, 0, st>>>(e->M.kv, e->M.gst, e->d_out, e->d_in, e->B, e->lam,
-                                           e->M.P, e->ent_part);
+  k_entropy<<<grid, 256, 0, st>>>(e->M.kv, e->M.gst, e->d_out, e->d_in, e->B, e->lam, e->M.P,
+                                  e->ent_part);
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
-  HIPCHK(hipMemcpyAsync(e->h_ent_part, e->ent_part, sizeof(long) * 2 * kEntropyBlocks,
-                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(e->h_ent_part, e->ent_part, sizeof(long) * 2 * grid, hipMemcpyDeviceToHost,
+                        st));
   HIPCHK(hipStreamSynchronize(st));
-  prof_add(e, USAP_K_ENTROPY, ev_ms(e, 0, 1), 1, 8.0 * e->B + 16.0 * kEntropyBlocks);
+  prof_add(e, USAP_K_ENTROPY, ev_ms(e, 0, 1), 1, 8.0 * e->B + 16.0 * grid);
   i128 t = 0;
-  for (int i = 0; i < kEntropyBlocks; ++i)
+  for (int i = 0; i < grid; ++i)
     t += mk128(e->h_ent_part[2 * i], (unsigned long)e->h_ent_part[2 * i + 1]);
   const i128 data = -t;
   if (hi) *hi = (int64_t)(data >> 64);
