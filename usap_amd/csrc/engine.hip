@@ -2235,6 +2235,7 @@ struct usap_engine {
   // profiling (HIP events on the engine's stream)
   bool prof = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool all_direct = false;  // every row of M direct-mapped (set by each rebuild)
   usap_kernel_prof kp[USAP_NKERNELS] = {};
   std::vector<usap_phase> phase_log;
 };
@@ -2470,6 +2471,10 @@ int rebuild_M(usap_engine* e) {
   DevSpM& m = e->M;
   m.total = tot;
   m.slack = e->p.slab_slack;
+  // every row direct-mapped (cap B each; a hashed row's cap is below B/8):
+  // no insert probes, so no row is ever queued or an increment parked until
+  // the next rebuild, and k_grow only has the batch's error flags to fold
+  e->all_direct = tot == (int64_t)B * B;
   if (3 * m.total > m.P) {  // two thirds of the pool is headroom for rows that grow
     dfree(e, m.kv);
     const int64_t P = 3 * m.total + (int64_t)4 * B + 4096;
@@ -3201,7 +3206,8 @@ int prof_end(usap_engine* e, size_t executed) {
 
 // Launches k_nodal + k_apply + k_grow for nodes [beg, end), batch bidx of the
 // sweep (random stream sweep->stream0 + bidx).
-int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t ev_slot) {
+int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t ev_slot,
+                 bool grow = true) {
   const int32_t B = e->B;
   hipStream_t st = e->stream;
   NodalArgs a;
@@ -3285,7 +3291,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
     return set_err(e, USAP_ESTATE, "profiled batch larger than its stamp slot");
   k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
   k_apply<<<grid + 1, kWave * kNodalWaves, 0, st>>>(b);  // block 0: the update step
-  k_grow<<<kGrowBlocks, 256, 0, st>>>(ga);
+  if (grow) k_grow<<<kGrowBlocks, 256, 0, st>>>(ga);
   HIPCHK(hipGetLastError());
   return USAP_OK;
 }
@@ -3351,7 +3357,10 @@ int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, int32_
       for (int32_t i = next; i < last; ++i, ++slot) {
         const int32_t beg = i * batch;
         const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
-        if (int rc = launch_batch(e, beg, end, i, slot)) return rc;
+        // with every row direct, the round's last k_grow folds the error
+        // flags of all its batches (they only report inconsistent M)
+        const bool grow = !e->all_direct || e->prof || i == last - 1;
+        if (int rc = launch_batch(e, beg, end, i, slot, grow)) return rc;
       }
     }
     HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, st));
