@@ -2206,6 +2206,8 @@ struct usap_engine {
   BlockPart* part = nullptr;  // per-workgroup shares of the running batch
   Sweep* sweep = nullptr;     // device sweep state
   Sweep* h_sweep = nullptr;   // pinned host copy
+  Sweep* h_chunk[2] = {nullptr, nullptr};  // sweep state after alternate launch chunks
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   int32_t* prop_r = nullptr;  // old label of each batch node
   unsigned long long* tstamp = nullptr;  // profiled batch launches: workgroup stamps
   size_t tstamp_n = 0;                   // its length
@@ -2663,12 +2665,15 @@ int usap_destroy(usap_engine* e) {
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   dfree(e, e->tstamp);
   pin_put(e->h_sweep, sizeof(Sweep));
+  for (int i = 0; i < 2; ++i) pin_put(e->h_chunk[i], sizeof(Sweep));
   dfree(e, e->big_off);
   dfree(e, e->big_tab);
   dfree(e, e->ent_part);
   pin_put(e->h_ent_part, sizeof(long) * 2 * kEntropyBlocks);
   for (int i = 0; i < kNumSnap; ++i) dfree(e, e->snap_labels[i]);
   for (hipEvent_t& ev : e->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t& ev : e->chunk_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (e->stream) {
     (void)hipStreamSynchronize(e->stream);
@@ -2846,13 +2851,16 @@ int alloc_state(usap_engine* e) {
   TRY(dalloc(e, &e->ent_part, 2 * kEntropyBlocks));
   e->h_sweep = (Sweep*)pin_get(sizeof(Sweep));
   e->h_ent_part = (long*)pin_get(sizeof(long) * 2 * kEntropyBlocks);
-  if (!e->h_sweep || !e->h_ent_part) return set_err(e, USAP_ENOMEM, "hipHostMalloc");
+  for (int i = 0; i < 2; ++i) e->h_chunk[i] = (Sweep*)pin_get(sizeof(Sweep));
+  if (!e->h_sweep || !e->h_ent_part || !e->h_chunk[0] || !e->h_chunk[1])
+    return set_err(e, USAP_ENOMEM, "hipHostMalloc");
   hipStream_t st = e->stream;
   HIPCHK(hipMemsetAsync(e->status, 0, sizeof(Status), st));
   HIPCHK(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
   HIPCHK(hipMemsetAsync(e->M.gst, 0, sizeof(GrowState), st));
   std::memset(e->h_sweep, 0, sizeof(Sweep));
   for (hipEvent_t& ev : e->ev) HIPCHK(hipEventCreate(&ev));
+  for (hipEvent_t& ev : e->chunk_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   return USAP_OK;
 }
 
@@ -3343,6 +3351,8 @@ int sweep_graph(usap_engine* e, int32_t batch, int32_t nb) {
 // device state already in e->sweep, one host synchronisation at the end;
 // rebuilds M and resumes when a batch overflowed a row slab.  Stops early
 // when the sweep's convergence test fires.
+constexpr int32_t kChunk = 16;  // batches per launch chunk of a sweep that may converge
+
 int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, int32_t* rebuilds) {
   hipStream_t st = e->stream;
   Sweep& hs = *e->h_sweep;
@@ -3353,14 +3363,33 @@ int run_sweep(usap_engine* e, int32_t batch, int32_t first, int32_t last, int32_
     } else {
       if (e->prof)
         if (int rc = prof_begin(e, (size_t)(last - next), batch)) return rc;
+      // A sweep whose convergence test is active stops at a batch nobody
+      // knows in advance, and the batches launched after it exit at once
+      // (a few us each).  Such a sweep goes out in chunks of kChunk batches,
+      // one chunk ahead of the host: before the host waits for chunk c's
+      // state, chunk c + 1 is queued, so the device never idles and at most
+      // one chunk is launched past the stop.
+      const bool chunked = !e->prof && hs.check && last - next > 2 * kChunk;
       size_t slot = 0;
-      for (int32_t i = next; i < last; ++i, ++slot) {
-        const int32_t beg = i * batch;
-        const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
-        // with every row direct, the round's last k_grow folds the error
-        // flags of all its batches (they only report inconsistent M)
-        const bool grow = !e->all_direct || e->prof || i == last - 1;
-        if (int rc = launch_batch(e, beg, end, i, slot, grow)) return rc;
+      int32_t c = 0;
+      for (int32_t i0 = next; i0 < last; ++c) {
+        const int32_t upto = chunked ? std::min<int32_t>(last, i0 + kChunk) : last;
+        for (int32_t i = i0; i < upto; ++i, ++slot) {
+          const int32_t beg = i * batch;
+          const int32_t end = std::min<int64_t>((int64_t)beg + batch, e->N);
+          // with every row direct, the chunk's last k_grow folds the error
+          // flags of all its batches (they only report inconsistent M)
+          const bool grow = !e->all_direct || e->prof || i == upto - 1;
+          if (int rc = launch_batch(e, beg, end, i, slot, grow)) return rc;
+        }
+        i0 = upto;
+        if (!chunked) break;
+        HIPCHK(hipMemcpyAsync(e->h_chunk[c & 1], e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(e->chunk_ev[c & 1], st));
+        if (c > 0) {
+          HIPCHK(hipEventSynchronize(e->chunk_ev[(c - 1) & 1]));
+          if (e->h_chunk[(c - 1) & 1]->stop != 0) break;
+        }
       }
     }
     HIPCHK(hipMemcpyAsync(&hs, e->sweep, sizeof(Sweep), hipMemcpyDeviceToHost, st));
