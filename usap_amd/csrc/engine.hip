@@ -834,6 +834,24 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
 // stores: one atomic per workgroup on a shared word costs more than the
 // kernel's own tail.  Per batch slot: k_nodal's starts and ends (G each),
 // k_apply's (G + 1 each), k_grow's (kGrowBlocks each).
+// The per-block counters and flags a rebuild starts from, zeroed in one
+// launch (degrees, edge counts, sizes, key counts, growth marks, parked
+// counts: B entries each; build_flags[0..2]).
+constexpr int kClearArrays = 8;
+struct ClearArgs {
+  int32_t* p[kClearArrays];
+  int32_t n;
+  int32_t* flags;
+};
+
+__global__ void k_clear_build(ClearArgs c) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int a = 0; a < kClearArrays; ++a)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += stride)
+      c.p[a][i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 3) c.flags[threadIdx.x] = 0;
+}
+
 __global__ void k_stamp_init(unsigned long long* t, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -2434,11 +2452,15 @@ int rebuild_M(usap_engine* e) {
   const int32_t B = e->B;
   hipStream_t st = e->stream;
   ev_rec(e, 0);
-  HIPCHK(hipMemsetAsync(e->d_out, 0, sizeof(int32_t) * B, st));
-  HIPCHK(hipMemsetAsync(e->d_in, 0, sizeof(int32_t) * B, st));
-  HIPCHK(hipMemsetAsync(e->ecnt_out, 0, sizeof(int32_t) * B, st));
-  HIPCHK(hipMemsetAsync(e->ecnt_in, 0, sizeof(int32_t) * B, st));
-  HIPCHK(hipMemsetAsync(e->bsize, 0, sizeof(int32_t) * B, st));
+  {
+    ClearArgs c;
+    int32_t* arr[kClearArrays] = {e->d_out, e->d_in, e->ecnt_out, e->ecnt_in,
+                                  e->bsize, e->M.nkeys, e->M.gmark, e->M.gpend};
+    for (int i = 0; i < kClearArrays; ++i) c.p[i] = arr[i];
+    c.n = B;
+    c.flags = e->build_flags;
+    k_clear_build<<<std::max(1u, std::min(blocks_for(B, 256), 1024u)), 256, 0, st>>>(c);
+  }
   if (e->N > 0) {
     const unsigned g = std::min<unsigned>(blocks_for(e->N, 256), B <= kDegLds ? 256u : 4096u);
     k_degrees<<<g, 256, 0, st>>>(gview(e), e->labels, B, e->d_out, e->d_in, e->ecnt_out,
@@ -2462,7 +2484,6 @@ int rebuild_M(usap_engine* e) {
   const bool maybe_single = B == e->N && e->E > 0;
   int32_t nonsingle = 1;
   if (maybe_single) {
-    HIPCHK(hipMemsetAsync(e->build_flags + 1, 0, sizeof(int32_t), st));
     k_count_nonsingle<<<std::min<unsigned>(blocks_for(B, 256), 1024u), 256, 0, st>>>(
         B, e->bsize, e->build_flags + 1);
     HIPCHK(hipMemcpyAsync(&nonsingle, e->build_flags + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -2485,16 +2506,12 @@ int rebuild_M(usap_engine* e) {
   }
   if (m.total > 0) k_fill<<<std::min<int64_t>(blocks_for(m.total, 256), 8192), 256, 0, st>>>(
       m.kv, m.total);
-  HIPCHK(hipMemsetAsync(m.nkeys, 0, sizeof(int32_t) * B, st));
-  HIPCHK(hipMemsetAsync(m.gmark, 0, sizeof(int32_t) * B, st));
-  HIPCHK(hipMemsetAsync(m.gpend, 0, sizeof(int32_t) * B, st));
   k_reset_grow<<<1, 1, 0, st>>>(m.gst, (unsigned long long)m.total);
   if (m.bs_rows < B) {
     dfree(e, m.bs);
     if (int rc = dalloc(e, &m.bs, (size_t)B * kNBK)) return rc;
     m.bs_rows = B;
   }
-  HIPCHK(hipMemsetAsync(e->build_flags, 0, sizeof(int32_t), st));
   if (e->E > 0) {
     if (maybe_single && nonsingle == 0)
       k_build_single<<<blocks_for(e->N, kNodalWaves), kWave * kNodalWaves, 0, st>>>(
