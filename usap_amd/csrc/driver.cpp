@@ -85,23 +85,23 @@ int nodal_phase(usap_engine* e, const usap_params& p, int32_t batch, int32_t B, 
   int rc = USAP_OK;
   std::fill(itr_dS.begin(), itr_dS.end(), 0.0);
   for (int32_t itr = 0; itr < p.max_nodal_itr; ++itr) {
-    // prepare_batch
-    auto t0 = clk::now();
-    S = 0.0;
-    if (itr >= W - 1)
-      if ((rc = usap_entropy(e, &S, nullptr, nullptr))) return rc;
-    st.ms_entropy += ms_since(t0);
+    // prepare_batch: the sweep's starting S (the entropy of the current
+    // state, only needed by the convergence test) is computed on the device
+    // ahead of the sweep's first batch
+    const bool test = itr >= W - 1;
+    const double no_S = 0.0;
     double mean = 0;
     if (itr >= W - 1) {
       for (int32_t i = itr - W + 1; i < itr; ++i) mean += itr_dS[i];
       mean /= (double)(W - 1);
     }
     st.sweeps++;
-    t0 = clk::now();
+    const auto t0 = clk::now();
     // fetch_batch / nodal_move / update / check_batch (graph_parallel.hpp:397-468),
     // the update step and its convergence test run on the device
     usap_sweep sw{};
-    if ((rc = usap_nodal_sweep(e, batch, nodal_stream, S, mean, thr, itr >= W - 1, &sw)))
+    if ((rc = usap::nodal_sweep(e, batch, nodal_stream, test ? nullptr : &no_S, mean, thr, test,
+                                &sw)))
       return rc;
     nodal_stream += (uint64_t)sw.batches;
     st.batches += sw.batches;

@@ -1730,6 +1730,31 @@ __device__ double fx_to_double_dev(i128 v) {
   return neg ? -d : d;
 }
 
+// A sweep's starting S from k_entropy's per-workgroup partials, stored in
+// the device sweep state: the exact int128 sum, then model + data in the same
+// IEEE double operations as usap_entropy on the host.
+__global__ __launch_bounds__(256) void k_entropy_to_sweep(const long* __restrict__ part,
+                                                          int32_t nparts, double model, Sweep* sw) {
+  __shared__ long s_hi[4];
+  __shared__ unsigned long s_lo[4];
+  i128 t = 0;
+  for (int32_t i = threadIdx.x; i < nparts; i += blockDim.x)
+    t += (i128)(((u128)(unsigned long)part[2 * i] << 64) | (u128)(unsigned long)part[2 * i + 1]);
+  t = wave_sum128(t);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    s_hi[w] = (long)(t >> 64);
+    s_lo[w] = (unsigned long)(u128)t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    i128 tot = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i)
+      tot += (i128)(((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i]);
+    sw->S = model + fx_to_double_dev(-tot);
+  }
+}
+
 // graph_parallel.hpp:433-458 after the moves of one batch (sum: exact sum of
 // the batch's accepted finite dS, ninf: accepted -inf ones): cumulative dS of
 // the sweep, S += dS_sum, the moving-average convergence test.  Same IEEE
@@ -3024,16 +3049,28 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
   return USAP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// k_entropy over the pool in use: about four slots per thread, at most
+// kEntropyBlocks workgroups (a scan of 1 GB pools at B = 250k is bound by the
+// loads in flight).  Returns the grid (the number of partials).
+int launch_entropy(usap_engine* e) {
+  const int64_t work = std::max<int64_t>(e->M.P, e->B);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(kEntropyBlocks, blocks_for(work, 1024)));
+  k_entropy<<<grid, 256, 0, e->stream>>>(e->M.kv, e->M.gst, e->d_out, e->d_in, e->B, e->lam,
+                                         e->M.P, e->ent_part);
+  return grid;
+}
+}  // namespace
+
+extern "C" {
+
 int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
   if (int rc = check_engine(e)) return rc;
   hipStream_t st = e->stream;
-  // about four pool slots per thread, at most kEntropyBlocks workgroups (a
-  // scan of 1 GB pools at B = 250k is bound by the loads in flight)
-  const int64_t work = std::max<int64_t>(e->M.P, e->B);
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(kEntropyBlocks, blocks_for(work, 1024)));
   ev_rec(e, 0);
-  k_entropy<<<grid, 256, 0, st>>>(e->M.kv, e->M.gst, e->d_out, e->d_in, e->B, e->lam, e->M.P,
-                                  e->ent_part);
+  const int grid = launch_entropy(e);
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
   HIPCHK(hipMemcpyAsync(e->h_ent_part, e->ent_part, sizeof(long) * 2 * grid, hipMemcpyDeviceToHost,
@@ -3482,11 +3519,34 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
 
 int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, double mean,
                      double thr, int32_t check, usap_sweep* out) {
+  return usap::nodal_sweep(e, batch, stream0, &S, mean, thr, check, out);
+}
+
+}  // extern "C"
+
+namespace usap {
+
+// usap_nodal_sweep; with S null the sweep's starting S is the entropy of the
+// current state, computed on the device between the state copy and the first
+// batch (no host wait for it).
+int nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, const double* S, double mean,
+                double thr, int32_t check, usap_sweep* out) {
   if (int rc = check_engine(e)) return rc;
   if (batch < 1 || !out) return USAP_EINVAL;
+  double S0 = 0.0;
+  if (!S && e->prof) {  // profiled: the entropy scan is timed on its own
+    if (int rc = usap_entropy(e, &S0, nullptr, nullptr)) return rc;
+    S = &S0;
+  }
   const int32_t nb = (int32_t)(((int64_t)e->N + batch - 1) / batch);
-  init_sweep(e, stream0, S, mean, thr, check);
+  init_sweep(e, stream0, S ? *S : 0.0, mean, thr, check);
   HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
+  if (!S) {
+    const int grid = launch_entropy(e);
+    k_entropy_to_sweep<<<1, 256, 0, e->stream>>>(e->ent_part, grid,
+                                                 usap_model_S(e->N, e->E, e->B), e->sweep);
+    HIPCHK(hipGetLastError());
+  }
   int32_t rebuilds = 0;
   if (int rc = run_sweep(e, batch, 0, nb, &rebuilds)) return rc;
   const Sweep& hs = *e->h_sweep;
@@ -3505,6 +3565,10 @@ int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, 
   out->rebuilds = rebuilds;
   return USAP_OK;
 }
+
+}  // namespace usap
+
+extern "C" {
 
 int usap_nodal_sum_fx(usap_engine* e, int64_t* hi, uint64_t* lo, int32_t* inf) {
   if (int rc = check_engine(e)) return rc;

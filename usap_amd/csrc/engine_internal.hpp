@@ -11,6 +11,10 @@ namespace usap {
 const usap_graph* engine_host_graph(const usap_engine* e);
 // Per-phase records of the running / last partition (usap_partition_log).
 std::vector<usap_phase>& engine_phase_log(usap_engine* e);
+// usap_nodal_sweep; with S null the sweep starts from the entropy of the
+// current state, computed on the device (no host wait for it).
+int nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, const double* S, double mean,
+                double thr, int32_t check, usap_sweep* out);
 const usap_params* engine_params(const usap_engine* e);
 int32_t engine_rebuilds(const usap_engine* e);
 // Rows moved to a larger slab between batches since the engine was created.
