@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, as the checker.  The product never imports it.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -47,6 +48,11 @@ class OrcParams(C.Structure):
         ("seed", C.c_uint64),
         ("threads", C.c_int32),
     ]
+
+
+class OrcPhase(C.Structure):
+    _fields_ = [("B", C.c_int64), ("S", C.c_double), ("sweeps", C.c_int64),
+                ("batches", C.c_int64), ("moves", C.c_int64)]
 
 
 class OrcSnap(C.Structure):
@@ -99,6 +105,12 @@ def lib():
                                         C.POINTER(OrcSnap), C.POINTER(C.c_int64)]
         L.orc_partition.argtypes = [C.c_void_p, _i32p, _f64p]
         L.orc_partition.restype = C.c_int32
+        L.orc_partition_log.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.orc_partition_log.restype = C.c_int32
+        L.orc_nodal_phase_scripted.argtypes = [C.POINTER(OrcParams), C.c_int32, C.c_int32,
+                                               C.c_int32, _f64p, C.c_int64, _i64p, _u64p,
+                                               _i32p, _i32p, C.c_int64, _f64p, _f64p]
+        L.orc_nodal_phase_scripted.restype = C.c_int
         L.orc_data_S_double.argtypes = [C.c_void_p]
         L.orc_data_S_double.restype = C.c_double
         L.orc_model_S.argtypes = [C.c_int64, C.c_int64, C.c_int64]
@@ -151,6 +163,26 @@ def golden_update(slots, cur, rate=0.5):
     btm = C.c_int64(0)
     rc = lib().orc_golden_update(arr, OrcSnap(*cur), rate, C.byref(nxt), C.byref(btm))
     return rc, [(a.B, a.S, a.id) for a in arr], (nxt.B, nxt.S, nxt.id), btm.value
+
+
+def nodal_phase_scripted(N, batch, isf, S_start, sums, moves=None, **params):
+    """The oracle's nodal-phase loop over scripted inputs: S_start[i] = MDL at
+    sweep i's start; sums[j] = batch j's accepted-dS sum (a Python int in
+    fixed point, or -math.inf).  Returns (stats dict, itr_dS)."""
+    p = default_params(**params)
+    hi = np.array([0 if s == -math.inf else (int(s) >> 64) for s in sums], np.int64)
+    lo = np.array([0 if s == -math.inf else (int(s) & ((1 << 64) - 1)) for s in sums], np.uint64)
+    inf = np.array([-1 if s == -math.inf else 0 for s in sums], np.int32)
+    mv = np.ascontiguousarray(moves if moves is not None else np.zeros(len(sums)), np.int32)
+    out = np.zeros(6, np.float64)
+    itr = np.zeros(max(p.max_nodal_itr, 1), np.float64)
+    rc = lib().orc_nodal_phase_scripted(C.byref(p), N, batch, isf,
+                                        np.ascontiguousarray(S_start, np.float64), len(S_start),
+                                        hi, lo, inf, mv, len(sums), out, itr)
+    if rc != 0:
+        raise ValueError("script ran out")
+    keys = ["sweeps", "batches", "proposals", "moves", "converged", "S_last"]
+    return dict(zip(keys, out.tolist())), itr
 
 
 class Oracle:
@@ -273,3 +305,10 @@ class Oracle:
         keys = ["S", "phases", "sweeps", "batches", "nodal_proposals", "moves",
                 "merge_proposals", "error"]
         return B, lab, dict(zip(keys, stats.tolist()))
+
+    def partition_log(self):
+        """Per-phase records of the last partition(): B, S, sweeps, batches, moves."""
+        n = lib().orc_partition_log(self.st, 0, None)
+        arr = (OrcPhase * max(n, 1))()
+        lib().orc_partition_log(self.st, n, arr)
+        return [{k: getattr(a, k) for k, _ in OrcPhase._fields_} for a in arr[:n]]

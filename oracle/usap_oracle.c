@@ -137,6 +137,9 @@ struct orc_state {
   int32_t* best_s;
   i128* best_dS;
   int32_t cap_B; /* allocation size for per-block arrays */
+  /* per-phase record of the last orc_partition */
+  orc_phase* plog;
+  int32_t plog_n, plog_cap;
 };
 
 /* sgn * T(m, x, y) */
@@ -327,6 +330,7 @@ void orc_destroy(orc_state* st) {
   free(st->row_w);
   free(st->col_idx);
   free(st->col_w);
+  free(st->plog);
   free(st);
 }
 
@@ -982,6 +986,166 @@ int orc_golden_update(orc_snap slots[3], orc_snap cur, double rate, orc_snap* ne
 }
 
 /* ------------------------------------------------------------------------ */
+/* The nodal phase, graph_parallel.hpp:381-478                               */
+/* ------------------------------------------------------------------------ */
+
+/* What one nodal phase needs from the partition state: the MDL at a sweep's
+ * start (prepare_batch, :383) and one batch (nodal_move + update, :406-458:
+ * proposals against the frozen state, then the accepted moves applied),
+ * returning the exact sum of the batch's accepted dS and its move count. */
+typedef struct {
+  void* ctx;
+  double (*entropy)(void* ctx);
+  void (*batch)(void* ctx, int32_t beg, int32_t end, uint64_t stream, dsacc* sum,
+                int32_t* moves);
+} nodal_ops;
+
+typedef struct {
+  int32_t sweeps;
+  int64_t batches, proposals, moves;
+  int converged;
+  double S_last; /* S after the last executed batch */
+} nodal_result;
+
+/* Sweeps of contiguous batches [beg, beg + batch) until the moving-average
+ * test fires or max_nodal_itr sweeps (graph_parallel.hpp:381-478):
+ *   prepare_batch :381-395  S = MDL; mean of itr_delta_entropy over the
+ *                           avg_window - 1 sweeps before this one (itr >=
+ *                           window - 1); the sweep's dS accumulator zeroed
+ *   update        :433-458  dS_sum = the sweep's accumulator, cumulative over
+ *                           its batches so far; itr_delta_entropy[itr] =
+ *                           dS_sum; S += dS_sum (cumulative);
+ *                           converged when itr >= window - 1 and
+ *                           -mean < thr * S, thr = thr2 if isf else thr1
+ *   check_batch   :460-468  next batch while beg < N
+ *   check_itr     :470-478  next sweep while itr < max_nodal_itr
+ * Batch j of the phase uses random stream *stream + j; *stream advances by
+ * the batches run.  itr_dS has max_nodal_itr entries. */
+static void nodal_phase(const orc_params* p, int32_t N, int32_t batch, int isf,
+                        const nodal_ops* ops, uint64_t* stream, double* itr_dS,
+                        nodal_result* res) {
+  const int32_t W = p->avg_window;
+  memset(res, 0, sizeof(*res));
+  memset(itr_dS, 0, sizeof(double) * (size_t)p->max_nodal_itr);
+  for (int32_t itr = 0; itr < p->max_nodal_itr; ++itr) {
+    double S = ops->entropy(ops->ctx);
+    double mean = 0;
+    if (itr >= W - 1) {
+      for (int32_t i = itr - W + 1; i < itr; ++i) mean += itr_dS[i];
+      mean /= (double)(W - 1);
+    }
+    dsacc cum = {0, 0};
+    res->sweeps += 1;
+    for (int32_t beg = 0; beg < N; beg += batch) {
+      const int32_t end = beg + batch > N ? N : beg + batch;
+      dsacc sum = {0, 0};
+      int32_t moves = 0;
+      ops->batch(ops->ctx, beg, end, (*stream)++, &sum, &moves);
+      res->batches += 1;
+      res->proposals += end - beg;
+      res->moves += moves;
+      if (sum.inf)
+        acc_inf(&cum, sum.inf);
+      else
+        cum.fx += sum.fx;
+      const double dS_sum = acc_to_double(&cum);
+      itr_dS[itr] = dS_sum;
+      S += dS_sum;
+      res->S_last = S;
+      if (itr >= W - 1) {
+        const double thr = isf ? p->thr2 : p->thr1;
+        if (-mean < thr * S) {
+          res->converged = 1;
+          return;
+        }
+      }
+    }
+  }
+}
+
+static double state_entropy(void* ctx) {
+  double S;
+  orc_entropy((const orc_state*)ctx, NULL, NULL, &S);
+  return S;
+}
+
+static void state_batch(void* ctx, int32_t beg, int32_t end, uint64_t stream, dsacc* sum,
+                        int32_t* moves) {
+  int64_t hi;
+  uint64_t lo;
+  int32_t sinf;
+  orc_nodal_batch((orc_state*)ctx, beg, end, stream, NULL, NULL, NULL, NULL, NULL, &hi, &lo,
+                  &sinf, moves);
+  sum->inf = sinf;
+  sum->fx = ((i128)hi << 64) | (i128)lo;
+}
+
+typedef struct {
+  const double* S_start;
+  int64_t n_S;
+  const int64_t* hi;
+  const uint64_t* lo;
+  const int32_t* inf;
+  const int32_t* moves;
+  int64_t n_batch;
+  int64_t sweep;
+  int bad;
+} script_ctx;
+
+static double script_entropy(void* vp) {
+  script_ctx* c = (script_ctx*)vp;
+  if (c->sweep >= c->n_S) {
+    c->bad = 1;
+    return 0.0;
+  }
+  return c->S_start[c->sweep++];
+}
+
+static void script_batch(void* vp, int32_t beg, int32_t end, uint64_t stream, dsacc* sum,
+                         int32_t* moves) {
+  script_ctx* c = (script_ctx*)vp;
+  (void)beg;
+  (void)end;
+  if ((int64_t)stream >= c->n_batch) {
+    c->bad = 1;
+    sum->fx = 0;
+    sum->inf = 0;
+    *moves = 0;
+    return;
+  }
+  sum->fx = ((i128)c->hi[stream] << 64) | (i128)c->lo[stream];
+  sum->inf = c->inf[stream];
+  *moves = c->moves ? c->moves[stream] : 0;
+}
+
+int32_t orc_partition_log(const orc_state* st, int32_t cap, orc_phase* out) {
+  for (int32_t i = 0; i < st->plog_n && i < cap && out; ++i) out[i] = st->plog[i];
+  return st->plog_n;
+}
+
+int orc_nodal_phase_scripted(const orc_params* p, int32_t N, int32_t batch, int32_t isf,
+                             const double* S_start, int64_t n_S, const int64_t* sum_hi,
+                             const uint64_t* sum_lo, const int32_t* sum_inf,
+                             const int32_t* moves, int64_t n_batch, double* out,
+                             double* itr_dS) {
+  script_ctx c = {S_start, n_S, sum_hi, sum_lo, sum_inf, moves, n_batch, 0, 0};
+  nodal_ops ops = {&c, script_entropy, script_batch};
+  nodal_result r;
+  uint64_t stream = 0;
+  double* dS = (double*)calloc((size_t)p->max_nodal_itr, sizeof(double));
+  nodal_phase(p, N, batch, isf, &ops, &stream, dS, &r);
+  if (itr_dS) memcpy(itr_dS, dS, sizeof(double) * (size_t)p->max_nodal_itr);
+  free(dS);
+  out[0] = r.sweeps;
+  out[1] = (double)r.batches;
+  out[2] = (double)r.proposals;
+  out[3] = (double)r.moves;
+  out[4] = r.converged;
+  out[5] = r.S_last;
+  return c.bad ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* partition(), graph_parallel.hpp:322-512                                   */
 /* ------------------------------------------------------------------------ */
 
@@ -992,7 +1156,6 @@ int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats) {
   int32_t N = st->g.N;
   int32_t batch = p->nodal_batch > 0 ? p->nodal_batch : N / p->num_batch_nodal_update;
   if (batch < 1) batch = 1;
-  int32_t W = p->avg_window;
   double* itr_dS = (double*)calloc((size_t)p->max_nodal_itr + 1, sizeof(double));
   int32_t** saved = (int32_t**)calloc(ORC_MAX_SAVED, sizeof(int32_t*));
   int64_t Btm = (int64_t)floor((double)st->B * p->reduction_rate);
@@ -1003,6 +1166,7 @@ int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats) {
     slots[i].id = -1;
   }
   double nphase = 0, nsweep = 0, nbatch = 0, nprop = 0, nmoves = 0, nmerge = 0, err = 0;
+  st->plog_n = 0;
   uint64_t merge_stream = 0, nodal_stream = 0;
   double S = 0;
   int32_t result = -1;
@@ -1015,48 +1179,30 @@ int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats) {
       break;
     }
     nphase += 1;
-    memset(itr_dS, 0, sizeof(double) * (size_t)(p->max_nodal_itr + 1));
-    for (int32_t itr = 0; itr < p->max_nodal_itr; ++itr) {
-      /* prepare_batch */
-      orc_entropy(st, NULL, NULL, &S);
-      double mean = 0;
-      if (itr >= W - 1) {
-        for (int32_t i = itr - W + 1; i < itr; ++i) mean += itr_dS[i];
-        mean /= (double)(W - 1);
-      }
-      int isf = isfinite(slots[0].S) && isfinite(slots[1].S) && isfinite(slots[2].S);
-      dsacc cum = {0, 0};
-      int converged = 0;
-      nsweep += 1;
-      for (int32_t beg = 0; beg < N; beg += batch) {
-        int32_t end = beg + batch > N ? N : beg + batch;
-        int64_t hi;
-        uint64_t lo;
-        int32_t sinf, moves;
-        orc_nodal_batch(st, beg, end, nodal_stream++, NULL, NULL, NULL, NULL, NULL, &hi, &lo,
-                        &sinf, &moves);
-        nbatch += 1;
-        nprop += end - beg;
-        nmoves += moves;
-        if (sinf)
-          acc_inf(&cum, sinf);
-        else
-          cum.fx += ((i128)hi << 64) | (i128)lo;
-        double dS_sum = acc_to_double(&cum);
-        itr_dS[itr] = dS_sum;
-        S += dS_sum;
-        if (itr >= W - 1) {
-          double thr = isf ? p->thr2 : p->thr1;
-          if (-mean < thr * S) {
-            converged = 1;
-            break;
-          }
-        }
-      }
-      if (converged) break;
-    }
+    /* the nodal phase (prepare_move :374-379, then :381-478); isf is the
+     * golden-section bracket's state, fixed during the phase */
+    const int isf = isfinite(slots[0].S) && isfinite(slots[1].S) && isfinite(slots[2].S);
+    nodal_ops ops = {st, state_entropy, state_batch};
+    nodal_result nr;
+    nodal_phase(p, N, batch, isf, &ops, &nodal_stream, itr_dS, &nr);
+    nsweep += nr.sweeps;
+    nbatch += (double)nr.batches;
+    nprop += (double)nr.proposals;
+    nmoves += (double)nr.moves;
     /* prepare_for_next */
     orc_entropy(st, NULL, NULL, &S);
+    if (st->plog_n == st->plog_cap) {
+      st->plog_cap = st->plog_cap ? 2 * st->plog_cap : 64;
+      st->plog = (orc_phase*)realloc(st->plog, sizeof(orc_phase) * (size_t)st->plog_cap);
+    }
+    {
+      orc_phase* ph = &st->plog[st->plog_n++];
+      ph->B = st->B;
+      ph->S = S;
+      ph->sweeps = nr.sweeps;
+      ph->batches = nr.batches;
+      ph->moves = nr.moves;
+    }
     if (phase >= ORC_MAX_SAVED) {
       err = -6;
       break;

@@ -161,6 +161,30 @@ int orc_golden_update(orc_snap slots[3], orc_snap cur, double rate, orc_snap* ne
 #define ORC_NSTATS 8
 int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats);
 
+/* One merge phase of the last orc_partition: B after the merge, the MDL at
+ * the phase's end, its nodal sweeps, batches and accepted moves. */
+typedef struct {
+  int64_t B;
+  double S;
+  int64_t sweeps, batches, moves;
+} orc_phase;
+/* Copies up to cap phase records of the last orc_partition; returns the count. */
+int32_t orc_partition_log(const orc_state* st, int32_t cap, orc_phase* out);
+
+/* The nodal phase of orc_partition (graph_parallel.hpp:381-478) driven by a
+ * script instead of a partition state, so that its loop semantics can be
+ * checked against hand-derived cases: S_start[i] is the MDL the i-th sweep
+ * starts from, and batch j of the phase returns the accepted-dS sum
+ * (sum_hi/lo[j], fixed point; sum_inf[j] 0 or -1) and moves[j] (nullable).
+ * out[6] = sweeps, batches, proposals, moves, converged, S after the last
+ * batch; itr_dS (nullable, [max_nodal_itr]) = itr_delta_entropy.  Returns -1
+ * if the script ran out, else 0. */
+int orc_nodal_phase_scripted(const orc_params* p, int32_t N, int32_t batch, int32_t isf,
+                             const double* S_start, int64_t n_S, const int64_t* sum_hi,
+                             const uint64_t* sum_lo, const int32_t* sum_inf,
+                             const int32_t* moves, int64_t n_batch, double* out,
+                             double* itr_dS);
+
 /* Textbook double-precision data_S = -sum m*log(m/(d_out*d_in)) (no fixed
  * point) for cross-checks. */
 double orc_data_S_double(const orc_state* st);

@@ -412,3 +412,225 @@ def test_oracle_thread_count_invariance(orc, usap):
     for B, lab, st in out[1:]:
         assert B == out[0][0] and st == out[0][2]
         np.testing.assert_array_equal(lab, out[0][1])
+
+
+# ---------------------------------------------------------------------------
+# merge proposal law, graph_parallel.hpp:774-865
+# ---------------------------------------------------------------------------
+
+def _merge_law(M, r):
+    """Exact law of the block a merge proposal of r draws
+    (graph_parallel.hpp:774-865), from the dense M:
+      k = sum_i M[r][i] + M[i][r]; k == 0: s uniform on [0, B) (r included);
+      else u ~ (M[r][u] + M[u][r]) / k, then with probability B / (d_a[u] + B)
+      a uniform draw whose hit on r moves to r + 1 (B wraps to 0), else
+      s ~ (M[u][s] + M[s][u]) over s != r, or the same uniform-not-r draw when
+      that weight is 0."""
+    B = M.shape[0]
+    w_r = (M[r] + M[:, r]).astype(float)
+    k = w_r.sum()
+    if k == 0:
+        return np.full(B, 1.0 / B)
+    not_r = np.full(B, 1.0 / B)
+    not_r[(r + 1) % B] += not_r[r]
+    not_r[r] = 0.0
+    da = M.sum(1) + M.sum(0)
+    p = np.zeros(B)
+    for u in range(B):
+        if w_r[u] == 0:
+            continue
+        pu = B / (da[u] + B)
+        q = (M[u] + M[:, u]).astype(float)
+        q[r] = 0.0
+        branch = not_r if q.sum() == 0 else q / q.sum()
+        p += (w_r[u] / k) * (pu * not_r + (1 - pu) * branch)
+    return p
+
+
+def _merge_draws(o, r, draws, stream=3):
+    counts = None
+    for idx in range(draws):
+        s, _ = o.merge_propose_one(r, idx, stream)
+        if counts is None:
+            counts = np.zeros(o.B, np.int64)
+        counts[s] += 1
+    return counts
+
+
+def _chi2_ok(counts, p):
+    from scipy.stats import chisquare
+    n = counts.sum()
+    zero = p == 0
+    assert np.all(counts[zero] == 0), (counts, p)
+    return chisquare(counts[~zero], p[~zero] * n).pvalue > 1e-4
+
+
+def _graph_of(usap, edges, N):
+    s = np.array([e[0] for e in edges], np.int32)
+    d = np.array([e[1] for e in edges], np.int32)
+    return usap.Graph.from_edges(s, d, N)
+
+
+@pytest.mark.parametrize("block_size", [1024, 4])
+def test_merge_proposal_law_random(orc, usap, block_size):
+    """Chi-square of the proposed merge target against the exact law on a
+    random labelled graph (dense-M and edge-list paths)."""
+    g = random_graph(90, 500, seed=14)
+    B = 7
+    lab = np.random.default_rng(15).integers(0, B, g.N).astype(np.int32)
+    o = _fresh(orc, g, lab, B, block_size=block_size)
+    M = o.M_dense()
+    for r in (0, 3, 6):
+        counts = _merge_draws(o, r, 12000)
+        assert _chi2_ok(counts, _merge_law(M, r)), (r, counts)
+
+
+def test_merge_proposal_law_edge_cases(orc, usap):
+    """Hand-built blocks for the branches of the law:
+      block 0: no edges at all (k = 0): s uniform on all B blocks, r included;
+      block 1: self-loops plus in-edges from block 4;
+      block 2: its only neighbour block is 3, whose only neighbour is 2: the
+               multinomial over u = 3 has no weight outside r, so s falls back
+               to uniform-not-r (r + 1 doubled);
+      block 4 (the last): a uniform-not-r draw that hits r wraps to block 0."""
+    # nodes: 0,1 -> block 0 (isolated); 2,3 -> block 1 (self-loops);
+    # 4 -> block 2, 5 -> block 3 (2 <-> 3 only); 6,7 -> block 4 with 4 <-> 1
+    edges = [(2, 2), (3, 3), (2, 2), (4, 5), (5, 4), (4, 5), (6, 7), (7, 2), (6, 3)]
+    g = _graph_of(usap, edges, 8)
+    lab = np.array([0, 0, 1, 1, 2, 3, 4, 4], np.int32)
+    B = 5
+    o = _fresh(orc, g, lab, B)
+    M = o.M_dense()
+    for r in range(B):
+        counts = _merge_draws(o, r, 8000)
+        assert _chi2_ok(counts, _merge_law(M, r)), (r, counts, _merge_law(M, r))
+    # block 0 (k = 0): every block, r included, equally likely
+    p0 = _merge_law(M, 0)
+    np.testing.assert_allclose(p0, np.full(B, 0.2))
+    # block 2: u = 3 always; 3's weight outside 2 is 0 -> uniform-not-2 (3 doubled)
+    np.testing.assert_allclose(_merge_law(M, 2), [0.2, 0.2, 0.0, 0.4, 0.2])
+
+
+def test_merge_proposal_law_pure_selfloops_and_two_blocks(orc, usap):
+    """A block whose only edges are self-loops (the multinomial over u = r
+    excludes r and has weight 0, so s is uniform-not-r: r + 1 twice as likely),
+    and B = 2 (every proposal of r with k > 0 is the other block)."""
+    g = _graph_of(usap, [(0, 0), (1, 1), (2, 3), (3, 4), (4, 2), (5, 5), (2, 5)], 6)
+    lab = np.array([0, 0, 1, 1, 1, 2], np.int32)
+    o = _fresh(orc, g, lab, 3)
+    M = o.M_dense()
+    p = _merge_law(M, 0)
+    np.testing.assert_allclose(p, [0.0, 2 / 3, 1 / 3])
+    assert _chi2_ok(_merge_draws(o, 0, 9000), p)
+    # B = 2: block 0 = {0, 1} with self-loops, block 1 the rest
+    lab2 = np.array([0, 0, 1, 1, 1, 1], np.int32)
+    o2 = _fresh(orc, g, lab2, 2)
+    for r in (0, 1):
+        counts = _merge_draws(o2, r, 500)
+        assert counts[r] == 0 and counts[1 - r] == 500
+
+
+def test_merge_proposal_law_distinguishes_mistakes(orc, usap):
+    """The law above rejects the plausible misreadings of :774-865 on a graph
+    built to separate them.  Blocks 0 = {0}, 1 = {1}, 2 = {2, 3} with edges
+    1 -> 0 and 2 <-> 3: a proposal of r = 1 always picks u = 0, takes the
+    uniform branch with probability 3 / (1 + 3), and otherwise finds u's only
+    weight on r, so it always ends in uniform-not-r: s = 2 (r + 1, doubled) with
+    probability 2/3 and s = 0 with 1/3.  A redraw instead of the r + 1 wrap
+    gives 1/2 each; a multinomial that keeps r would propose r itself."""
+    g = _graph_of(usap, [(1, 0), (2, 3), (3, 2)], 4)
+    lab = np.array([0, 1, 2, 2], np.int32)
+    o = _fresh(orc, g, lab, 3)
+    M = o.M_dense()
+    np.testing.assert_allclose(_merge_law(M, 1), [1 / 3, 0.0, 2 / 3])
+    counts = _merge_draws(o, 1, 6000)
+    assert _chi2_ok(counts, _merge_law(M, 1))
+    from scipy.stats import chisquare
+    assert chisquare(counts[[0, 2]], [3000, 3000]).pvalue < 1e-10
+    assert counts[1] == 0
+
+
+# ---------------------------------------------------------------------------
+# nodal-phase loop, graph_parallel.hpp:381-478 (hand-derived scripts)
+# ---------------------------------------------------------------------------
+
+ONE = 1 << 48  # 1.0 in the fixed point
+
+
+def _script(N=8, batch=2, isf=0, S_start=None, sums=None, **kw):
+    from oracle import oracle as O
+    return O.nodal_phase_scripted(N, batch, isf, S_start, sums, **kw)
+
+
+def test_nodal_loop_converges_on_third_sweep():
+    """4 batches of -1 per sweep: itr_delta_entropy = -1, -2, -3, -4 within a
+    sweep (cumulative), -4 at its end.  No test on sweeps 0 and 1; on sweep 2
+    mean = (-4 + -4) / 2 = -4 and after its first batch S = 10000 - 1 = 9999:
+    4 < 5e-4 * 9999 = 4.9995 fires.  3 sweeps, 9 batches, 18 proposals."""
+    st, itr = _script(S_start=[10000.0] * 3, sums=[-ONE] * 12)
+    assert st["sweeps"] == 3 and st["batches"] == 9 and st["converged"] == 1
+    assert st["proposals"] == 18
+    assert list(itr[:3]) == [-4.0, -4.0, -1.0]
+    assert st["S_last"] == 9999.0
+
+
+def test_nodal_loop_cumulative_S_and_threshold_switch():
+    """Same script with the golden-section bracket complete (isf): threshold
+    1e-4, so 4 < 0.9999 fails and the sweeps run on.  Sweep 2 moves nothing
+    (sums 0): on sweep 3 mean = (-4 + 0) / 2 = -2, 2 < 1e-4 * S fails; on
+    sweep 4 mean = 0 and 0 < 1e-4 * S fires after its first batch.  S within a
+    sweep adds the cumulative sum after every batch (S_start 10000, sums -1 x 4
+    -> 10000 - 1 - 2 - 3 - 4 = 9990 after sweep 0)."""
+    sums = [-ONE] * 8 + [0] * 12
+    st, itr = _script(isf=1, S_start=[10000.0] * 5, sums=sums)
+    assert st["sweeps"] == 5 and st["batches"] == 17 and st["converged"] == 1
+    assert list(itr[:5]) == [-4.0, -4.0, 0.0, 0.0, 0.0]
+    st0, _ = _script(isf=1, S_start=[10000.0], sums=[-ONE] * 4, max_nodal_itr=1)
+    assert st0["S_last"] == 9990.0 and st0["converged"] == 0 and st0["batches"] == 4
+    # without the bracket (threshold 5e-4) the same script stops on sweep 2
+    st1, _ = _script(isf=0, S_start=[10000.0] * 5, sums=sums)
+    assert st1["sweeps"] == 3 and st1["batches"] == 9
+
+
+def test_nodal_loop_window_is_the_two_previous_sweeps():
+    """Batch sums -25 x 4 in sweep 0 (sweep total -100), -0.25 x 4 after (total
+    -1), S 10000, threshold 5e-4 (-> about 5).  Sweep 2: mean = (-100 - 1) / 2
+    = -50.5 does not fire; sweep 3: mean = (-1 - 1) / 2 = -1 fires after its
+    first batch: 4 sweeps, 13 batches.  A window that took the running sweep's
+    (still zero) entry would fire on sweep 2; one reaching back three sweeps
+    would still hold the -100 on sweep 3."""
+    sums = [-25 * ONE] * 4 + [-ONE // 4] * 12
+    st, itr = _script(S_start=[10000.0] * 4, sums=sums)
+    assert list(itr[:4]) == [-100.0, -1.0, -1.0, -0.25]
+    assert st["sweeps"] == 4 and st["batches"] == 13 and st["converged"] == 1
+    assert st["S_last"] == 10000.0 - 0.25
+
+
+def test_nodal_loop_minus_infinity():
+    """An accepted -inf dS (ref mode) makes the sweep's cumulative sum -inf:
+    S = -inf fails the test for the rest of that sweep, and the mean stays
+    -inf while the sweep is in the window.  -inf in sweep 1, zeros after:
+    sweeps 2 and 3 cannot fire (mean -inf), sweep 4 (mean 0) fires after its
+    first batch: 5 sweeps, 17 batches."""
+    sums = [-ONE] * 4 + [0, 0, -math.inf, 0] + [0] * 12
+    st, itr = _script(S_start=[10000.0] * 5, sums=sums)
+    assert itr[1] == -math.inf and itr[0] == -4.0
+    assert st["sweeps"] == 5 and st["batches"] == 17 and st["converged"] == 1
+
+
+def test_nodal_loop_ragged_last_batch_and_cap():
+    """N = 7, batch 2: batches [0,2) [2,4) [4,6) [6,7), 7 proposals a sweep;
+    nothing converges with S = 0 (0 < 0 is false): max_nodal_itr sweeps."""
+    st, _ = _script(N=7, batch=2, S_start=[0.0] * 6, sums=[-ONE] * 24, max_nodal_itr=6)
+    assert st["sweeps"] == 6 and st["batches"] == 24 and st["proposals"] == 42
+    assert st["converged"] == 0
+
+
+def test_nodal_loop_no_test_before_window():
+    """Nothing moves at all: the test may first run on sweep 2 (itr >=
+    window - 1), where mean 0 < 5e-4 * S fires: 3 sweeps, 9 batches; with a
+    window of 4 it first runs on sweep 3."""
+    st, _ = _script(S_start=[100.0] * 4, sums=[0] * 16)
+    assert st["sweeps"] == 3 and st["batches"] == 9
+    st4, _ = _script(S_start=[100.0] * 4, sums=[0] * 16, avg_window=4)
+    assert st4["sweeps"] == 4 and st4["batches"] == 13
