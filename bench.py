@@ -45,60 +45,162 @@ def workload(name):
     return g, params, desc
 
 
-def cpu_baseline(g, params, phase_log, final_labels, final_B, seconds):
-    """The oracle (clean-room CPU restatement at the reference's complexity,
-    16 worker threads like the reference's default executor) timed on bounded
-    samples of nodal proposals at two states of the same run: after the first
-    merge (largest B) and the final partition (smallest B).  The per-phase CPU
-    rate is interpolated in log B between the two, and the run's nodal
-    proposals are divided by the resulting CPU time."""
+def cpu_info():
+    """(CPUs this process may run on, CPU model name)."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return nproc, model
+
+
+def cpu_baseline(g, params, phase_log, seconds, threads=16):
+    """The oracle (clean-room CPU restatement at the reference's complexity:
+    O(B) dense scratch per proposal, M rebuilt per batch; `threads` workers
+    like the reference's default executor) timed on bounded samples at every
+    state of the halving chain the partition walks (B = N/2, N/4, ... down to
+    the final B, each reached on the GPU engine by merge rounds from the
+    identity start): merge proposals of a sample of blocks and nodal
+    proposals of a sample of nodes.  Every phase of the run is then priced
+    at the rates measured at its B (log-log interpolation between the two
+    nearest sampled states): merge proposals at the B they were made at,
+    nodal proposals at the phase's B.  The CPU partition time is the sum
+    (merge + nodal only: M builds, carry-outs and entropies are left out, in
+    the CPU's favour); `value` = the run's nodal proposals / that time."""
     from oracle.oracle import Oracle
     import usap_amd
-    threads = min(16, os.cpu_count() or 1)
+    K = params.get("K", 10)
+    final_B = min(p["B"] for p in phase_log)
     eng = usap_amd.Engine(g, **dict(params, verbose=0))
     eng.set_labels(np.arange(g.N, dtype=np.int32), g.N)
-    eng.merge_propose(0)
-    eng.merge_apply(g.N // 2)
-    first_labels, first_B = eng.labels(), eng.B
+    o = Oracle(g, threads=threads, seed=params["seed"])
+    states = []
+    stream = 0
+    while True:
+        B = eng.B
+        target = max(final_B, B // 2)
+        if target >= B:
+            break
+        eng.merge_propose(stream)
+        eng.merge_apply(B - target)
+        stream += 1
+        states.append((eng.B, eng.labels()))
+        if eng.B <= final_B:
+            break
     eng.close()
-    rates = {}
-    for tag, lab, B in (("first", first_labels, first_B), ("final", final_labels, final_B)):
-        o = Oracle(g, threads=threads, seed=params["seed"])
-        o.set_labels(lab, B)
-        n, el = 256, 0.0
+    per = seconds / max(1, 2 * len(states))
+
+    def timed(fn, n0, nmax):
+        n = n0
         while True:
             t = time.perf_counter()
-            o.nodal_eval(0, min(n, g.N), 0)
+            fn(n)
             el = time.perf_counter() - t
-            if el > seconds / 4 or n >= g.N:
-                break
-            n = min(g.N, int(n * max(2.0, (seconds / 4) / max(el, 1e-3))))
-        rates[tag] = (B, min(n, g.N) / el, min(n, g.N), el)
-        o.close()
-        log(f"[bench] cpu sample {tag}: B={B} {min(n, g.N)} proposals in {el:.2f}s "
-            f"-> {min(n, g.N) / el:.0f}/s ({threads} threads)")
-    (b1, r1, _, _), (b2, r2, _, _) = rates["first"], rates["final"]
+            if el > per / 2 or n >= nmax:
+                return n, el
+            n = min(nmax, int(n * max(2.0, (per / 2) / max(el, 1e-3))))
 
-    def rate(B):
-        if b1 == b2:
-            return r1
-        f = (math.log(B) - math.log(b2)) / (math.log(b1) - math.log(b2))
-        f = min(1.0, max(0.0, f))
-        return math.exp(math.log(r2) + f * (math.log(r1) - math.log(r2)))
+    rates = []
+    for B, lab in states:
+        o.set_labels(lab, B)
+        nm, em = timed(lambda m: o.merge_eval(0, m, 0), 16, B)
+        nn, en = timed(lambda n: o.nodal_eval(0, n, 0), 256, g.N)
+        rates.append((B, K * nm / em, nn / en))
+        log(f"[bench] cpu sample B={B}: {K * nm} merge proposals in {em:.2f}s, "
+            f"{nn} nodal proposals in {en:.2f}s ({threads} threads)")
+    o.close()
 
+    def rate(B, col):
+        pts = sorted(rates)
+        if B <= pts[0][0]:
+            return pts[0][col]
+        if B >= pts[-1][0]:
+            return pts[-1][col]
+        for (b1, *r1), (b2, *r2) in zip(pts, pts[1:]):
+            if b1 <= B <= b2:
+                if b1 == b2:
+                    return r1[col - 1]
+                f = (math.log(B) - math.log(b1)) / (math.log(b2) - math.log(b1))
+                return math.exp(math.log(r1[col - 1]) + f * (math.log(r2[col - 1]) -
+                                                             math.log(r1[col - 1])))
+        return pts[-1][col]
+
+    merge_s = sum(p["merge_proposals"] / rate(p["merge_proposals"] // K, 1) for p in phase_log)
+    nodal_s = sum(p["proposals"] / rate(p["B"], 2) for p in phase_log)
     props = sum(p["proposals"] for p in phase_log)
-    cpu_s = sum(p["proposals"] / rate(p["B"]) for p in phase_log)
+    nproc, model = cpu_info()
     return {
-        "value": props / cpu_s,
+        "value": props / (merge_s + nodal_s),
         "unit": "node-moves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle nodal proposals timed at B={b1} ({rates['first'][2]} proposals, "
-                   f"{rates['first'][1]:.0f}/s) and B={b2} ({rates['final'][2]} proposals, "
-                   f"{rates['final'][1]:.0f}/s); the run's {props} proposals over {len(phase_log)} "
-                   f"phases priced by log-B interpolation of those rates (extrapolated "
-                   f"nodal time {cpu_s:.0f}s)"),
+        "nproc": nproc,
+        "cpu_model": model,
+        "extrapolated": True,
+        "cpu_partition_s": round(merge_s + nodal_s, 1),
+        "cpu_merge_s": round(merge_s, 1),
+        "cpu_nodal_s": round(nodal_s, 1),
+        "rates": [{"B": b, "merge_proposals_per_s": round(rm, 1),
+                   "nodal_proposals_per_s": round(rn, 1)} for b, rm, rn in rates],
+        "sample": (f"oracle at the reference's complexity, {threads} threads, timed on bounded "
+                   f"samples (merge proposals of the first blocks, nodal proposals of the first "
+                   f"nodes) at the {len(rates)} states of the halving chain B={rates[0][0]}.."
+                   f"{rates[-1][0]}; the run's {sum(p['merge_proposals'] for p in phase_log)} "
+                   f"merge and {props} nodal proposals over {len(phase_log)} phases priced at "
+                   f"the measured per-phase rates (log-log interpolation in B): extrapolated CPU "
+                   f"partition time {merge_s + nodal_s:.0f}s (merge {merge_s:.0f}s, nodal "
+                   f"{nodal_s:.0f}s; M builds and entropies not counted)"),
     }
+
+
+def cpu_full_runs(names=("c1", "c2"), threads=16):
+    """Whole partitions (reference defaults: SCC init, ref dS, nodal batch
+    N/4) of smaller workloads on the oracle, at `threads` workers and at every
+    CPU this process may use (the faster is reported), beside the engine's
+    partition of the same graph (graph resident, one warm-up).  Both must give
+    the same B and S (bit-exact arithmetic, shared random stream)."""
+    from oracle.oracle import Oracle
+    import usap_amd
+    nproc, _ = cpu_info()
+    out = {}
+    for name in names:
+        cfg = usap_amd.CONFIGS[name]
+        g = usap_amd.config_graph(name)
+        params = dict(seed=cfg["seed"], init=0)
+        eng = usap_amd.Engine(g, **params)
+        eng.partition()
+        t = time.perf_counter()
+        lab, st = eng.partition()
+        gpu_s = time.perf_counter() - t
+        eng.close()
+        best = None
+        for th in sorted({threads, min(nproc, 64)}):
+            o = Oracle(g, threads=th, **params)
+            t = time.perf_counter()
+            o.set_labels(*o.scc())
+            B, olab, ost = o.partition()
+            el = time.perf_counter() - t
+            o.close()
+            log(f"[bench] cpu full {name}: {el:.2f}s at {th} threads (B={B})")
+            if best is None or el < best[0]:
+                best = (el, th, B, ost["S"], olab)
+        el, th, B, S, olab = best
+        out[name] = {"N": g.N, "E": int(g.E), "cpu_s": round(el, 3), "cpu_threads": th,
+                     "gpu_s": round(gpu_s, 4), "speedup": round(el / gpu_s, 1),
+                     "B": B, "S": S,
+                     "same_partition": bool(B == st["B"] and S == st["S"] and
+                                            np.array_equal(lab, olab)),
+                     "nmi": round(usap_amd.nmi(g.truth, lab), 4)}
+    return out
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -117,17 +219,23 @@ def copy_calibration(nbytes=1 << 30, reps=3):
 
 
 def measured_traffic(config):
-    """HBM bytes per k_nodal launch for this workload, as recorded in the
-    counter summary under profiles/ (None when there is none)."""
+    """HBM bytes per k_nodal launch for this workload and where the figure
+    comes from, as recorded in the counter summary under profiles/ ((None,
+    None) when there is none)."""
     path = os.path.join(ROOT, "profiles", f"k_nodal_{config}_traffic.txt")
+    val, src = None, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
-                    return float(line.split("=")[1])
+                    val = float(line.split("=")[1])
+                elif line.startswith("measured_at"):
+                    src = line.split("=", 1)[1].strip()
     except OSError:
         pass
-    return None
+    if val is not None:
+        src = f"profiles/k_nodal_{config}_traffic.txt" + (f" ({src})" if src else "")
+    return val, src
 
 
 def batched_bench(n_graphs=256, workers=0, replay=0):
@@ -179,7 +287,7 @@ def batched_bench(n_graphs=256, workers=0, replay=0):
 
 
 def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interval_s=0.05,
-                 check=4, replay=0, final_partition=True):
+                 check=4, replay=0, final_partition=True, repartition_growth=2.0):
     """The served stream: n_graphs DCSBMs (N log-uniform in [1000, 200000])
     arriving by a Poisson process at each offered rate, each delivered as 10
     increments of 10% of its shuffled edges, interval_s apart; stage 1
@@ -197,7 +305,8 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
     for rate in rates:
         arr = sv.poisson_arrivals(len(graphs), rate, seed=5)
         labs, per, st = sv.serve(graphs, arr, params, stages=10, interval_s=interval_s,
-                                 workers=workers, final_partition=final_partition)
+                                 workers=workers, final_partition=final_partition,
+                                 repartition_growth=repartition_growth)
         log(f"[bench] served at {rate} graphs/s: {st['graphs_per_s']:.2f} graphs/s, "
             f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms, resident <= "
             f"{st['max_resident']}, wall {st['wall_s']:.1f}s")
@@ -212,16 +321,36 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
     same = True
     for i in range(min(check, len(graphs))):
         lab1, per1, _ = sv.serve([graphs[i]], [0.0], [params[i]], stages=10, workers=1,
-                                 final_partition=final_partition)
+                                 final_partition=final_partition,
+                                 repartition_growth=repartition_growth)
         for labs, per in mixes:
             same &= bool(np.array_equal(labs[i], lab1[0]))
             same &= [r["S"] for r in per[i]] == [r["S"] for r in per1[0]]
+    # throughput at saturation against the plain form: every graph and every
+    # increment available at once, served one graph at a time (one worker)
+    # and with `workers` stages in flight; the per-stage MDL must agree
+    zero = np.zeros(len(graphs))
+    plain_l, plain_p, plain_st = sv.serve(graphs, zero, params, stages=10, interval_s=0.0,
+                                          workers=1, final_partition=final_partition,
+                                          repartition_growth=repartition_growth)
+    sat_l, sat_p, sat_st = sv.serve(graphs, zero, params, stages=10, interval_s=0.0,
+                                    workers=workers, final_partition=final_partition,
+                                    repartition_growth=repartition_growth)
+    same_mdl = all([r["S"] for r in a] == [r["S"] for r in b] for a, b in zip(plain_p, sat_p))
+    log(f"[bench] served at saturation: plain {plain_st['graphs_per_s']:.2f} graphs/s, "
+        f"{workers} in flight {sat_st['graphs_per_s']:.2f} graphs/s, same MDL {same_mdl}")
     keep = [p for p in points if p["graphs_per_s"] >= 0.9 * min(p["offered"], len(graphs))]
     last = mixes[-1]
     return {
         "metric": "served graphs/s",
         "value": max((p["graphs_per_s"] for p in keep), default=0.0),
         "points": points,
+        "saturated_graphs_per_s": round(sat_st["graphs_per_s"], 3),
+        "plain_graphs_per_s": round(plain_st["graphs_per_s"], 3),
+        "speedup_vs_plain": round(sat_st["graphs_per_s"] / max(plain_st["graphs_per_s"], 1e-12), 2),
+        "same_mdl_as_plain": bool(same_mdl),
+        "full_partitions_per_graph": round(float(np.mean([sum(r["full"] for r in p)
+                                                          for p in sat_p])), 2),
         "same_alone_and_in_mix": bool(same),
         "checked_alone": min(check, len(graphs)),
         "mean_final_nmi": round(float(np.mean([usap_nmi(g.truth, l)
@@ -230,8 +359,10 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
         "workload": (f"{len(graphs)} directed DCSBMs, N log-uniform in [1000, 200000], "
                      "round(N^0.4) planted blocks, p_in uniform in [0.55, 0.85], Poisson "
                      f"arrivals, 10 increments of 10% of the shuffled edges {interval_s}s apart; "
-                     "stage 1 partition (SCC init), stages 2-9 resume the chain, stage 10 "
-                     + ("partitions the whole graph" if final_partition else "resumes")),
+                     "stage 1 partition (SCC init); later stages append their increment on "
+                     f"the device and resume the chain, or partition afresh once the edges "
+                     f"reach {repartition_growth}x those of the last full partition"
+                     + ("; stage 10 partitions the whole graph" if final_partition else "")),
     }
 
 
@@ -246,8 +377,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the SCC-init partition and the batched workload")
     ap.add_argument("--graphs", type=int, default=256, help="batched workload size (c4)")
     ap.add_argument("--workers", type=int, default=0, help="graphs in flight (c4; 0 = autotune)")
     ap.add_argument("--replay", action="store_true",
@@ -296,8 +429,8 @@ def main():
     el = time.perf_counter() - t0
     props = sum(s["nodal_proposals"] for s in stats)
     value = props / el
-    # kernel times and byte tallies from one more partition, outside the timed
-    # region: the per-launch events slow the run down
+    # kernel times and byte counts from one more partition, outside the timed
+    # region (the byte counts run kernels of their own)
     eng.profile(1)
     t = time.perf_counter()
     _, pst = eng.partition()
@@ -306,13 +439,14 @@ def main():
     plog = eng.partition_log()
     assert pst["S"] == stats[-1]["S"] and pst["batches"] == stats[-1]["batches"]
 
-    kern = max(prof, key=lambda k: prof[k]["ms"])
-    if prof["nodal"]["ms"] > 0:
-        kern_roof = "nodal"  # the nodal proposal kernel carries the byte tally
-    else:
-        kern_roof = kern
-    kp = prof[kern_roof]
-    achieved = (kp["bytes"] / 1e9) / (kp["ms"] / 1e3) if kp["ms"] > 0 else 0.0
+    kp = prof["nodal"]
+    secs = kp["ms"] / 1e3
+    achieved = kp["model_bytes"] / 1e9 / secs if secs > 0 else 0.0
+    own = kp["bytes"] / 1e9 / secs if secs > 0 else 0.0
+    launches = max(kp["launches"], 1)
+    mp = prof["merge"]
+    msecs = mp["ms"] / 1e3
+    traffic, traffic_src = measured_traffic(args.config)
     roofline = {
         "bound": "hbm",
         "kernel": "k_nodal",
@@ -320,10 +454,25 @@ def main():
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 5),
-        "traffic": measured_traffic(args.config),
-        "bytes_per_launch": kp["bytes"] / max(kp["launches"], 1),
-        "avg_launch_us": 1e3 * kp["ms"] / max(kp["launches"], 1),
+        "traffic": traffic,
+        "traffic_source": traffic_src,
+        "algorithmic_bytes": "A(Q) of the reference algorithm from the run's own batches",
+        "bytes_per_launch": kp["model_bytes"] / launches,
+        "avg_launch_us": 1e3 * kp["ms"] / launches,
         "launches": kp["launches"],
+        "own_tally": {
+            "achieved": round(own, 2),
+            "frac": round(own / PEAK_HBM_GBS, 5),
+            "bytes_per_launch": kp["bytes"] / launches,
+            "what": "bytes the factorised kernel itself must read and write",
+        },
+        "merge": {
+            "kernel": "k_merge",
+            "achieved": round(mp["model_bytes"] / 1e9 / msecs, 2) if msecs > 0 else 0.0,
+            "frac": round(mp["model_bytes"] / 1e9 / msecs / PEAK_HBM_GBS, 5) if msecs > 0 else 0.0,
+            "bytes_per_launch": mp["model_bytes"] / max(mp["launches"], 1),
+            "avg_launch_us": 1e3 * mp["ms"] / max(mp["launches"], 1),
+        },
     }
     out = {
         "metric": "node-moves/s",
@@ -355,11 +504,35 @@ def main():
         "cpu_baseline": None,
     }
     eng.close()
+    if not args.no_extra:
+        # the same graph from the reference's default start (SCC init)
+        eng = usap_amd.Engine(g, **dict(params, init=0))
+        t = time.perf_counter()
+        slab, sst = eng.partition()
+        eng.close()
+        out["scc_init"] = {"B": sst["B"], "S": sst["S"], "nmi": round(usap_amd.nmi(g.truth, slab), 5),
+                           "ms": round(1e3 * (time.perf_counter() - t), 1)}
+        log(f"[bench] SCC init: B={sst['B']} S={sst['S']:.1f} nmi={out['scc_init']['nmi']}")
+        out["planted"] = {"B": int(g.truth.max()) + 1,
+                          "S": planted_S(usap_amd, g, params)}
+        b = batched_bench(args.graphs, args.workers)
+        out["batched"] = {k: b[k] for k in ("value", "plain_graphs_per_s", "speedup_vs_plain",
+                                            "workers", "same_mdl_and_labels", "graphs",
+                                            "mean_nmi", "workload")}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, params, plog, lab, stats[-1]["B"],
-                                           args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(g, params, plog, args.cpu_seconds)
         out["vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        out["cpu_full_runs"] = cpu_full_runs()
     print(json.dumps(out), flush=True)
+
+
+def planted_S(usap_amd, g, params):
+    """MDL of the planted partition (the generator's truth labels)."""
+    eng = usap_amd.Engine(g, **params)
+    eng.set_labels(g.truth.astype(np.int32), int(g.truth.max()) + 1)
+    S = eng.entropy()[1]
+    eng.close()
+    return S
 
 
 if __name__ == "__main__":

@@ -75,8 +75,10 @@ typedef struct usap_params {
   int32_t verbose;                 /* print per-phase progress to stderr */
   int32_t slab_slack;              /* hashed block row capacity = pow2 >= slack*(out+in edges)+16; a row grows past 1/slack load (2; 1..16) */
   int32_t graphs;                  /* 1: replay each nodal sweep as a captured HIP graph (0 =
-                                      default; only with one engine per process: another
-                                      thread's HIP calls can invalidate a capture) */
+                                      default).  The capture is thread-local
+                                      (hipStreamCaptureModeThreadLocal) and the engine's own
+                                      thread makes no other HIP call while it lasts, so other
+                                      engines may run on other threads meanwhile */
 } usap_params;
 
 /* Partition statistics (usap_partition).  Times are host wall-clock ms. */
@@ -117,6 +119,26 @@ const char* usap_last_error(const usap_engine* e);
  * reference (one graph per Graph_P object, graph_parallel.hpp:61-78); for
  * callers that receive a graph as streamed edge increments. */
 int usap_set_graph(usap_engine* e, const usap_graph* g);
+
+/* Appends n edges src[i] -> dst[i] (weights w, NULL = 1; weights only for
+ * a weighted graph) after the graph's current edges, keeping the labels and
+ * B.  Only the increment crosses to the device: the CSR/CSC lists grow on
+ * the device, each node's new neighbours after its old ones in delivery
+ * order (the reference's neighbour order, graph_parallel.hpp:290-294), and
+ * M and the block degrees are rebuilt.  Equal to usap_set_graph with the
+ * CSR/CSC of the concatenated edge list (usap_build_csr).  No counterpart
+ * in the reference; for callers that receive a graph as streamed edge
+ * increments (its ingest, load_graph_from_tsv, graph_parallel.hpp:251-317,
+ * reads one whole file). */
+int usap_append_edges(usap_engine* e, int64_t n, const int32_t* src, const int32_t* dst,
+                      const int32_t* w);
+
+/* usap_create from an edge list in delivery order (0-based; w NULL =
+ * unweighted): the CSR/CSC are built on the device.  ≙ Graph_P ctor +
+ * load_graph_from_tsv's list building (graph_parallel.hpp:61-78, :251-317)
+ * on an edge list already in memory. */
+int usap_create_from_edges(int32_t N, int64_t E, const int32_t* src, const int32_t* dst,
+                           const int32_t* w, const usap_params* p, usap_engine** out);
 
 /* Set labels (all in [0,B)) and rebuild M and the block degrees.
  * ≙ labels assignment + _initialize_edge_counts (graph_parallel.hpp:703-759). */
@@ -221,6 +243,7 @@ typedef struct usap_phase {
   double k_build_ms;
   double k_grow_ms;
   int32_t rebuilds;    /* full M rebuilds in this phase */
+  int64_t merge_proposals; /* K x blocks before the merge */
 } usap_phase;
 
 /* Copies up to cap phase records of the last usap_partition; *n = count. */
@@ -271,6 +294,10 @@ typedef struct usap_serve_opts {
   int64_t hbm_budget;   /* device bytes for resident graphs by usap_footprint (0 = no limit) */
   int32_t final_partition; /* 1: the last stage runs a full partition of the whole graph
                               instead of resuming (the resumed chain keeps stage 1's B) */
+  double repartition_growth; /* > 1: a stage whose edges reach this multiple of the
+                                edges at the graph's last full partition runs a full
+                                partition (merges and the search over B) instead of
+                                resuming; 0 = never */
 } usap_serve_opts;
 
 typedef struct usap_serve_stage {  /* one per (graph, stage), graph-major */
@@ -285,6 +312,7 @@ typedef struct usap_serve_stage {  /* one per (graph, stage), graph-major */
   int64_t E;            /* edges in this stage */
   int64_t sweeps;
   int64_t nodal_proposals;
+  int32_t full;         /* 1: this stage ran a full partition */
 } usap_serve_stage;
 
 typedef struct usap_serve_stats {
@@ -304,10 +332,12 @@ typedef struct usap_serve_stats {
 int64_t usap_footprint(int32_t N, int64_t E);
 
 /* Serves n graphs: graph g arrives at graphs[g].arrival_s and its stage k
- * (1..stages) becomes ready interval_s*(k-1) later.  Stage 1 = usap_create
- * + usap_partition; stage k > 1 = usap_set_graph + usap_resume (nodal
- * streams from k << 40), so the engine stays resident between stages; with
- * final_partition the last stage is usap_set_graph + usap_partition.
+ * (1..stages) becomes ready interval_s*(k-1) later.  Stage 1 =
+ * usap_create_from_edges + usap_partition; stage k > 1 = usap_append_edges
+ * (the increment only) + usap_resume (nodal streams from k << 40), so the
+ * engine stays resident between stages; a stage runs usap_partition instead
+ * when the graph has grown by repartition_growth since its last full
+ * partition, and so does the last stage with final_partition.
  * Graphs are admitted in arrival order while their footprints fit
  * hbm_budget; up to `workers` ready stages of different graphs run at once,
  * oldest first.  Each graph's final labels go to labels + label_off[g];
@@ -319,7 +349,7 @@ int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts*
                usap_serve_stats* out);
 
 /* Picks the number of graphs in flight: partitions a sample of `sample`
- * graphs spread over the size range (0 = 24) with 2, 4, 8, ... workers (at
+ * graphs spread over the size range (0 = 24) with 1, 2, 4, 8, ... workers (at
  * most max_workers, 0 = 16) while graphs/s improves by 2% or more.  tried /
  * rates (nullable, >= 16 entries) receive each trial's workers and graphs/s. */
 int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* params,
@@ -328,9 +358,20 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
 
 /* Kernel profile: time on the engine's stream (nodal, apply and grow: from
  * the launch's first workgroup start to its last workgroup end, stamped in
- * the kernel; the others: events around their launches), launches, and the
- * algorithmic bytes the kernel moved (reads + writes the algorithm needs,
- * each counted once; 0 where not tallied). */
+ * the kernel; the others: events around their launches), launches, the
+ * bytes the kernel moved by its own algorithm (reads + writes it needs, each
+ * counted once; 0 where not tallied), and, for the nodal and merge kernels,
+ * the compulsory bytes of the reference's algorithm for the same work, from
+ * the run's own counts: per nodal batch Q
+ *   A(Q) = sum_n (16 + 4 + 8 k_n + 12 [+ 4 k_n weighted])
+ *        + sum_{b in D(Q)} (16 + 8 nnzR(b) + 8 nnzC(b) + 12)
+ *        + sum_{n accepted} (4 + 16 k_n),
+ * D(Q) the distinct blocks among the batch's r_n, s_n, u_n (old block,
+ * proposed block, drawn neighbour's block; graph_parallel.hpp:990-1222 reads
+ * their rows and columns of M whole); per merge round
+ *   A = 2 (4 (B+1) + 8 nnz(M)) + 12 B + 12 B
+ * (graph_parallel.hpp:761-988, rows and columns of M read once).  Counting
+ * them runs extra kernels, so profiled times come from a separate run. */
 #define USAP_K_NODAL 0   /* nodal proposals */
 #define USAP_K_APPLY 1   /* accepted-move application */
 #define USAP_K_MERGE 2   /* merge proposals */
@@ -341,7 +382,8 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
 typedef struct usap_kernel_prof {
   int64_t launches;
   double ms;
-  double bytes;
+  double bytes;        /* this kernel's own algorithmic bytes */
+  double model_bytes;  /* the reference algorithm's compulsory bytes (nodal, merge; else 0) */
 } usap_kernel_prof;
 /* enable = 1: reset and start timing; 0: stop; -1: leave as is.  out
  * (nullable, [USAP_NKERNELS]) receives the totals since the last reset. */

@@ -101,6 +101,8 @@ def lib():
                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.orc_nodal_eval.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64,
                                      C.POINTER(C.c_int32)]
+        L.orc_merge_eval.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64]
+        L.orc_merge_eval.restype = None
         L.orc_golden_update.argtypes = [C.POINTER(OrcSnap), OrcSnap, C.c_double,
                                         C.POINTER(OrcSnap), C.POINTER(C.c_int64)]
         L.orc_partition.argtypes = [C.c_void_p, _i32p, _f64p]
@@ -297,6 +299,10 @@ class Oracle:
         a = C.c_int32()
         lib().orc_nodal_eval(self.st, beg, end, stream, C.byref(a))
         return a.value
+
+    def merge_eval(self, r0, r1, stream):
+        """The K merge proposals of blocks [r0, r1) (timing only)."""
+        lib().orc_merge_eval(self.st, r0, r1, stream)
 
     def partition(self):
         lab = np.zeros(self.g.N, np.int32)

@@ -581,11 +581,13 @@ typedef struct {
   orc_state* st;
   tscratch* ts;
   uint64_t stream;
+  int64_t first; /* block of item 0 */
 } merge_ctx;
 
-static void merge_item(void* vp, int tid, int64_t r) {
+static void merge_item(void* vp, int tid, int64_t i) {
   merge_ctx* c = (merge_ctx*)vp;
   orc_state* st = c->st;
+  const int64_t r = c->first + i;
   int32_t best = -1;
   i128 bestd = 0;
   for (int32_t idx = 0; idx < st->p.K; ++idx) {
@@ -607,12 +609,26 @@ void orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s, int64_t* d
   c.st = st;
   c.ts = scratch_alloc(st->p.threads, st->B);
   c.stream = stream;
+  c.first = 0;
   par_for(st->p.threads, st->B, merge_item, &c);
   scratch_free(c.ts, st->p.threads);
   for (int32_t r = 0; r < st->B; ++r) {
     if (best_s) best_s[r] = st->best_s[r];
     if (dS_hi) split128(st->best_dS[r], &dS_hi[r], &dS_lo[r]);
   }
+}
+
+void orc_merge_eval(orc_state* st, int32_t r0, int32_t r1, uint64_t stream) {
+  if (r0 < 0) r0 = 0;
+  if (r1 > st->B) r1 = st->B;
+  if (r1 <= r0) return;
+  merge_ctx c;
+  c.st = st;
+  c.ts = scratch_alloc(st->p.threads, st->B);
+  c.stream = stream;
+  c.first = r0;
+  par_for(st->p.threads, r1 - r0, merge_item, &c);
+  scratch_free(c.ts, st->p.threads);
 }
 
 /* ------------------------------------------------------------------------ */

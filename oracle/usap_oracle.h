@@ -113,6 +113,9 @@ void orc_merge_propose_one(orc_state* st, int32_t r, int32_t idx, uint64_t strea
 /* All blocks, K proposals each; best s and dS per block. */
 void orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s,
                      int64_t* dS_hi, uint64_t* dS_lo);
+/* The K proposals of blocks [r0, r1) only (best per block kept in the
+ * state, no output); times the CPU baseline on a bounded sample. */
+void orc_merge_eval(orc_state* st, int32_t r0, int32_t r1, uint64_t stream);
 /* Greedy carry out of B_to_merge merges over the last orc_block_merge result;
  * relabels and rebuilds M.  Returns 0, or -1 if the candidates run out. */
 int orc_carry_out(orc_state* st, int32_t B_to_merge);

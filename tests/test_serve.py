@@ -107,3 +107,29 @@ def test_serve_final_partition(usap):
     eng.close()
     np.testing.assert_array_equal(lab, labs[0])
     assert stt["S"] == per[0][2]["S"] and stt["B"] == per[0][2]["B"]
+
+
+@pytest.mark.gpu
+def test_serve_repartition_on_growth(usap):
+    """repartition_growth 2: a stage whose edges reach twice those of the
+    last full partition partitions afresh (stages 1, 2, 4, 8 of 10 here);
+    each such stage equals a full partition of its prefix alone, and the
+    stages between resume the chain."""
+    gs, params = _small()
+    g, p = gs[3], params[3]
+    labs, per, st = sv.serve([g], [0.0], [p], stages=10, workers=1, repartition_growth=2.0)
+    assert st["failed"] == 0
+    E = [int(np.ceil(g.E * k / 10)) for k in range(1, 11)]
+    want, last = [], None
+    for k, Ek in enumerate(E, 1):
+        full = k == 1 or Ek >= 2 * last
+        want.append(int(full))
+        if full:
+            last = Ek
+    assert [r["full"] for r in per[0]] == want
+    for k, rec in enumerate(per[0], 1):
+        if rec["full"]:
+            gk = usap.Graph.from_edges(g.src[:E[k - 1]], g.dst[:E[k - 1]], N=g.N)
+            with usap.Engine(gk, **p) as eng:
+                lab, stt = eng.partition()
+            assert stt["S"] == rec["S"] and stt["B"] == rec["B"], k

@@ -102,6 +102,7 @@ class UsapPhase(C.Structure):
         ("k_build_ms", C.c_double),
         ("k_grow_ms", C.c_double),
         ("rebuilds", C.c_int32),
+        ("merge_proposals", C.c_int64),
     ]
 
 
@@ -155,6 +156,7 @@ class UsapServeOpts(C.Structure):
         ("workers", C.c_int32),
         ("hbm_budget", C.c_int64),
         ("final_partition", C.c_int32),
+        ("repartition_growth", C.c_double),
     ]
 
 
@@ -173,6 +175,7 @@ class UsapServeStage(C.Structure):
         ("E", C.c_int64),
         ("sweeps", C.c_int64),
         ("nodal_proposals", C.c_int64),
+        ("full", C.c_int32),
     ]
 
     def as_dict(self):
@@ -199,7 +202,8 @@ class UsapServeStats(C.Structure):
 
 
 class UsapKernelProf(C.Structure):
-    _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double)]
+    _fields_ = [("launches", C.c_int64), ("ms", C.c_double), ("bytes", C.c_double),
+                ("model_bytes", C.c_double)]
 
 
 KERNELS = ["nodal", "apply", "merge", "entropy", "build", "grow"]
@@ -212,6 +216,9 @@ SIGNATURES = [
     ("usap_destroy", C.c_int, [_vp]),
     ("usap_last_error", C.c_char_p, [_vp]),
     ("usap_set_graph", C.c_int, [_vp, C.POINTER(UsapGraph)]),
+    ("usap_append_edges", C.c_int, [_vp, C.c_int64, _vp, _vp, _vp]),
+    ("usap_create_from_edges", C.c_int, [C.c_int32, C.c_int64, _vp, _vp, _vp,
+                                         C.POINTER(UsapParams), C.POINTER(_vp)]),
     ("usap_resume", C.c_int, [_vp, C.c_uint64, _vp, C.POINTER(UsapStats)]),
     ("usap_set_labels", C.c_int, [_vp, _i32p, C.c_int32]),
     ("usap_get_labels", C.c_int, [_vp, _i32p]),

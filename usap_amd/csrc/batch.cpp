@@ -7,8 +7,10 @@
 // machine, taking graphs from one shared queue ordered largest first, so the
 // GPU runs the kernels of several graphs concurrently while each graph's
 // results stay exactly what it gets alone (every engine is deterministic and
-// independent of the others).  Engine memory is stream ordered, so engines
-// come and go without synchronising the device.
+// independent of the others).  A worker's engines share the worker's one HIP
+// stream and its own stream-ordered memory pool: engines come and go without
+// synchronising the device, and no stream or pool is created or destroyed
+// per graph.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -39,14 +41,29 @@ int32_t run_pool(const std::vector<int32_t>& idx, const usap_graph* graphs,
   std::atomic<int32_t> next{0};
   std::atomic<int32_t> failed{0};
   auto work = [&]() {
+    // this worker's stream and memory pool per device, shared by the engines
+    // it creates one after another
+    std::vector<std::pair<int32_t, usap::WorkerCtx*>> ctxs;
+    auto ctx_for = [&](int32_t dev, usap::WorkerCtx** w) {
+      for (auto& c : ctxs)
+        if (c.first == dev) {
+          *w = c.second;
+          return USAP_OK;
+        }
+      const int rc = usap::worker_ctx_create(dev, w);
+      if (rc == USAP_OK) ctxs.emplace_back(dev, *w);
+      return rc;
+    };
     for (;;) {
       const int32_t k = next.fetch_add(1);
-      if (k >= m) return;
+      if (k >= m) break;
       const int32_t i = idx[k];
       usap_engine* e = nullptr;
       usap_stats st;
       std::memset(&st, 0, sizeof(st));
-      int rc = usap_create(&graphs[i], &params[i], &e);
+      usap::WorkerCtx* w = nullptr;
+      int rc = ctx_for(params[i].device, &w);
+      if (rc == USAP_OK) rc = usap::create_in(w, &graphs[i], &params[i], &e);
       if (rc == USAP_OK) rc = usap_partition(e, labels + label_off[i], &st);
       if (rc != USAP_OK)
         usap::set_detached_error("usap_partition_batch: graph " + std::to_string(i) + ": error " +
@@ -56,6 +73,7 @@ int32_t run_pool(const std::vector<int32_t>& idx, const usap_graph* graphs,
       if (stats) stats[i] = st;
       if (rc != USAP_OK) failed.fetch_add(1);
     }
+    for (auto& c : ctxs) usap::worker_ctx_destroy(c.second);
   };
   workers = std::max<int32_t>(1, std::min<int32_t>(workers, m));
   std::vector<std::thread> pool;
@@ -138,10 +156,11 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
   std::vector<int64_t> off(n + 1, 0);
   for (int32_t i = 0; i < n; ++i) off[i + 1] = off[i] + graphs[i].N;
   std::vector<int32_t> scratch((size_t)off[n]);
-  // double the worker count while graphs/s improves by at least 2%
+  // one graph at a time first (the plain form), then double the worker
+  // count while graphs/s improves by at least 2%
   double best = 0.0;
   int32_t best_w = 1, k = 0;
-  for (int32_t w = 2; w <= max_workers; w *= 2) {
+  for (int32_t w = 1; w <= max_workers; w *= 2) {
     const double t0 = now_ms();
     const int32_t failed =
         run_pool(idx, graphs, params, scratch.data(), off.data(), nullptr, w);

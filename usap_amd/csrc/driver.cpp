@@ -147,6 +147,7 @@ int usap_golden_update(int64_t* slot_B, double* slot_S, int64_t* slot_id, int64_
 int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
   if (!e) return USAP_EINVAL;
   const usap_graph* g = usap::engine_host_graph(e);
+  if (!g) return USAP_EHIP;
   const usap_params& p = *usap::engine_params(e);
   const int32_t N = g->N;
   usap_stats st{};
@@ -192,6 +193,7 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
     // wait for the proposals here so that ms_merge holds the merge kernel
     if ((rc = usap_merge_result_fx(e, nullptr, nullptr, nullptr))) break;
     st.merge_proposals += (int64_t)B * p.K;
+    ph.merge_proposals = (int64_t)B * p.K;
     st.ms_merge += ms_since(t0);
     t0 = clk::now();
     if ((rc = usap_merge_apply(e, (int32_t)B_to_merge, &B))) break;
@@ -282,6 +284,7 @@ int usap_partition(usap_engine* e, int32_t* labels_out, usap_stats* stats) {
 int usap_resume(usap_engine* e, uint64_t stream0, int32_t* labels_out, usap_stats* stats) {
   if (!e) return USAP_EINVAL;
   const usap_graph* g = usap::engine_host_graph(e);
+  if (!g) return USAP_EHIP;
   const usap_params& p = *usap::engine_params(e);
   usap_stats st{};
   int32_t B = 0;

@@ -904,6 +904,71 @@ __global__ void k_relabel(int32_t N, int32_t* labels, const int32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Graph ingest on the device: CSR (out-lists) and CSC (in-lists) extended by
+// an increment of edges.  Lists keep the reference's neighbour order -- a
+// node's neighbours in the order their edges were delivered
+// (graph_parallel.hpp:290-294) -- so node x's new list is its old list, then
+// its new edges in delivery order.  With cnt[x] new edges at x, the new
+// offsets are new_ptr[x] = old_ptr[x] + sum_{y<x} cnt[y]; old edge j of x
+// moves by that shift, and the j-th edge of the increment stably sorted by
+// x lands at old_ptr[x+1] + j.
+// ---------------------------------------------------------------------------
+
+__global__ void k_count_keys(int64_t n, const int32_t* __restrict__ key, int32_t* cnt) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[key[i]], 1);
+}
+
+// deg[x] = old list length + new edges of x, deg[N] = 0 (exclusive-scan input).
+__global__ void k_new_deg(int32_t N, const int64_t* __restrict__ old_ptr,
+                          const int32_t* __restrict__ cnt, int64_t* deg) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < N) deg[x] = old_ptr[x + 1] - old_ptr[x] + cnt[x];
+  if (x == N) deg[x] = 0;
+}
+
+// Old lists to their new places, one wavefront per node; owner (nullable)
+// records the node of each new slot (the CSR's edge sources).
+__global__ __launch_bounds__(256) void k_move_lists(int32_t N, const int64_t* __restrict__ old_ptr,
+                                                    const int64_t* __restrict__ new_ptr,
+                                                    const int32_t* __restrict__ old_idx,
+                                                    const int32_t* __restrict__ old_w,
+                                                    int32_t* new_idx, int32_t* new_w, int32_t* owner) {
+  const int64_t x = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (x >= N) return;
+  const int64_t o0 = old_ptr[x], o1 = old_ptr[x + 1], shift = new_ptr[x] - o0;
+  for (int64_t j = o0 + lane_id(); j < o1; j += kWave) {
+    new_idx[j + shift] = old_idx[j];
+    if (new_w) new_w[j + shift] = old_w[j];
+    if (owner) owner[j + shift] = (int32_t)x;
+  }
+}
+
+// The increment's edges (sorted stably by their node key) to their places:
+// sorted edge j of node x = key[j] goes to old_ptr[x + 1] + j.  other[] is
+// the edge's other end (the target for CSR, the source for CSC).
+__global__ void k_place_new(int64_t n, const int32_t* __restrict__ skey,
+                            const int32_t* __restrict__ sidx, const int64_t* __restrict__ old_ptr,
+                            const int32_t* __restrict__ other, const int32_t* __restrict__ w,
+                            int32_t* new_idx, int32_t* new_w, int32_t* owner) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t x = skey[j], i = sidx[j];
+    const int64_t pos = old_ptr[x + 1] + j;
+    new_idx[pos] = other[i];
+    if (new_w) new_w[pos] = w ? w[i] : 1;
+    if (owner) owner[pos] = x;
+  }
+}
+
+__global__ void k_iota(int64_t n, int32_t* v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------------------
 // Entropy, graph_parallel.hpp:1281-1319:
 // data_S = -(sum_ab L(M_ab) - sum_a L(d_out[a]) - sum_b L(d_in[b]))
 // ---------------------------------------------------------------------------
@@ -1236,6 +1301,19 @@ __global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
 constexpr int kEdgeRegs = 4;     // list entries per lane per pass (256 per wavefront)
 constexpr int kScrNodal = 1024;  // k_nodal's LDS scratch per wavefront (one key bucket)
 
+// Workgroups are dispatched to the GPU's 8 XCDs (each with its own L2) in
+// turn: workgroup b runs on XCD b % 8.  Block b of a batch kernel takes item
+// block xcd_block(b, G) instead of b, so the workgroups of one XCD cover
+// consecutive nodes, and the cache lines holding those nodes' CSR/CSC offsets,
+// lists and proposal outputs are fetched into one L2 instead of up to eight.
+// A bijection of [0, G).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t G) {
+  const int32_t per = G / kXcds, extra = G % kXcds;
+  const int32_t x = b % kXcds, j = b / kXcds;
+  return (x < extra ? x * (per + 1) : extra * (per + 1) + (x - extra) * per) + j;
+}
+
 // One workgroup's share of a batch, written by k_nodal and summed by block 0
 // of k_apply (no same-address atomics per node).
 struct BlockPart {
@@ -1272,6 +1350,7 @@ struct NodalArgs {
   int32_t* prop_inf;
   int32_t* prop_acc;
   int32_t* prop_r;
+  int32_t* prop_u;  // profiling: block of the drawn neighbour (-1: none), else null
   BlockPart* part;
   Status* status;
   Sweep* sweep;  // read, and stop_n written by block 0
@@ -1573,6 +1652,7 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
     a.prop_inf[item] = inf ? -1 : 0;
     a.prop_acc[item] = acc;
     a.prop_r[item] = r;
+    if (a.prop_u) a.prop_u[item] = u;
   }
   acc_out = acc;
   inf_out = inf ? 1 : 0;
@@ -1592,7 +1672,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     // updates stop meanwhile
     a.sweep->stop_n = a.sweep->stop;
   }
-  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + w;
+  const int64_t item = (int64_t)xcd_block(blockIdx.x, gridDim.x) * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
   long bytes = 0, abytes = 0;
   int32_t acc = 0, inf = 0;
@@ -1648,6 +1728,79 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
     P.ninf = ninf;
     if (a.tstamp) a.tstamp[a.tstride + blockIdx.x] = wall_clock64();
   }
+}
+
+// Profiling only: the compulsory bytes of the reference's algorithm for one
+// nodal batch, from the batch's own proposals.  graph_parallel.hpp:990-1222
+// reads, per node n, its list offsets (16), label (4), neighbour ids and
+// labels (8 k_n, + 4 k_n weights) and writes s and dS (12); per distinct
+// block b among the batch's old blocks r_n, proposed blocks s_n and drawn
+// neighbours' blocks u_n, row and column b of M whole (16 + 8 nnzR(b) +
+// 8 nnzC(b)) and its degrees (12); per accepted move, the label (4) and two
+// 8-byte read-modify-writes per list entry (16 k_n).  A block counts once per
+// batch: the first wavefront to raise its mark to the batch's stamp counts
+// the non-zeros of its block row (o > 0: row b, i > 0: column b).  One
+// wavefront per batch node; acc[0] += per-node bytes, acc[1] += block bytes.
+struct AqArgs {
+  GraphView g;
+  MView M;
+  int32_t beg, end;
+  const int32_t* prop_r;
+  const int32_t* prop_s;
+  const int32_t* prop_u;
+  const int32_t* prop_acc;
+  int32_t* mark;
+  int32_t stamp;
+  const Sweep* sweep;
+  unsigned long long* acc;
+};
+
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_aq(AqArgs a) {
+  const int lane = lane_id();
+  const int64_t item = (int64_t)blockIdx.x * kNodalWaves + threadIdx.x / kWave;
+  const int64_t n64 = (int64_t)a.beg + item;
+  if (n64 >= a.end || a.sweep->stop_n) return;  // uniform per wavefront
+  const int32_t n = (int32_t)n64;
+  const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
+  int32_t b = -1;
+  if (lane == 0) b = a.prop_r[item];
+  if (lane == 1) b = a.prop_s[item];
+  if (lane == 2) b = a.prop_u[item];
+  const bool first = b >= 0 && atomicMax(&a.mark[b], a.stamp) < a.stamp;
+  unsigned long long rows = __ballot(first);
+  unsigned long long blk = 0;
+  while (rows) {
+    const int l = __ffsll(rows) - 1;
+    rows &= rows - 1ull;
+    const int32_t bb = __shfl(b, l);
+    const int64_t o = a.M.off[bb];
+    const int32_t c = a.M.cap[bb];
+    long nz = 0;
+    for (int32_t j = lane; j < c; j += kWave) {
+      const Slot q = a.M.kv[o + j];
+      nz += (q.o > 0 ? 1 : 0) + (q.i > 0 ? 1 : 0);
+    }
+    blk += 16 + 8 * (unsigned long long)wave_sum(nz) + 12;
+  }
+  if (lane == 0) {
+    unsigned long long node = 16 + 4 + 8 * (unsigned long long)kc + 12;
+    if (a.g.out_w) node += 4 * (unsigned long long)kc;
+    if (a.prop_acc[item]) node += 4 + 16 * (unsigned long long)kc;
+    atomicAdd(&a.acc[0], node);
+    if (blk) atomicAdd(&a.acc[1], blk);
+  }
+}
+
+// Profiling only: 16 bytes per non-zero of M (its entry read from the row
+// and from the column structure of graph_parallel.hpp:761-988) into acc[0].
+__global__ __launch_bounds__(256) void k_nnz_bytes(const Slot* __restrict__ kv, int64_t P,
+                                                   unsigned long long* acc) {
+  long c = 0;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < P;
+       j += (int64_t)gridDim.x * blockDim.x)
+    c += kv[j].o > 0 ? 1 : 0;
+  c = wave_sum(c);
+  if (lane_id() == 0 && c) atomicAdd(acc, 16ull * (unsigned long long)c);
 }
 
 // M build when every block holds one node (the identity start, B == N): block
@@ -1933,7 +2086,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
     return;
   }
   const int w = threadIdx.x / kWave;
-  const int64_t item = (int64_t)(blockIdx.x - 1) * kNodalWaves + w;
+  const int64_t item = (int64_t)xcd_block(blockIdx.x - 1, gridDim.x - 1) * kNodalWaves + w;
   const int64_t n64 = (int64_t)a.beg + item;
   const bool in = n64 < a.end;
   const int32_t n = in ? (int32_t)n64 : a.beg;
@@ -2204,6 +2357,8 @@ struct DevSpM {
 struct usap_engine {
   usap_params p{};
   hipStream_t stream = nullptr;
+  bool own_stream = true;      // false: a worker context's stream, kept at destroy
+  hipMemPool_t pool = nullptr;  // a worker context's pool, else the device's default pool
   std::string err;
   // host copy of the graph
   int32_t N = 0;
@@ -2212,7 +2367,9 @@ struct usap_engine {
   std::vector<int64_t> h_out_ptr, h_in_ptr;
   std::vector<int32_t> h_out_idx, h_in_idx, h_out_w, h_in_w;
   usap_graph hg{};
+  bool host_stale = false;  // lists appended on the device since the host copy was made
   int64_t Wtot = 0;
+  int64_t lam_cap = 0;  // entries of the device lambda table
   // device graph
   int64_t *out_ptr = nullptr, *in_ptr = nullptr;
   int32_t *out_idx = nullptr, *in_idx = nullptr, *out_w = nullptr, *in_w = nullptr;
@@ -2252,6 +2409,13 @@ struct usap_engine {
   Sweep* h_chunk[2] = {nullptr, nullptr};  // sweep state after alternate launch chunks
   hipEvent_t chunk_ev[2] = {nullptr, nullptr};
   int32_t* prop_r = nullptr;  // old label of each batch node
+  // reference-algorithm byte counts (profiling): drawn neighbour's block per
+  // batch node, per-block batch stamps, device totals [nodal per-node,
+  // nodal per-block, merge non-zeros]
+  int32_t* prop_u = nullptr;
+  int32_t* aq_mark = nullptr;
+  unsigned long long* aq_acc = nullptr;
+  int32_t aq_stamp = 0;
   unsigned long long* tstamp = nullptr;  // profiled batch launches: workgroup stamps
   size_t tstamp_n = 0;                   // its length
   int32_t tstamp_G = 0;                  // k_nodal / k_apply workgroups per slot this round
@@ -2319,7 +2483,21 @@ int dalloc(usap_engine* e, T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
   e->epoch++;
-  hipError_t st = hipMallocAsync((void**)p, n * sizeof(T), e->stream);
+  hipError_t st = e->pool ? hipMallocFromPoolAsync((void**)p, n * sizeof(T), e->pool, e->stream)
+                          : hipMallocAsync((void**)p, n * sizeof(T), e->stream);
+  if (st != hipSuccess) {
+    *p = nullptr;
+    return set_err(e, USAP_ENOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(st));
+  }
+  return USAP_OK;
+}
+
+// dalloc for untyped scratch (rocPRIM temporary storage).
+int dalloc_raw(usap_engine* e, void** p, size_t bytes) {
+  *p = nullptr;
+  e->epoch++;
+  hipError_t st = e->pool ? hipMallocFromPoolAsync(p, bytes ? bytes : 1, e->pool, e->stream)
+                          : hipMallocAsync(p, bytes ? bytes : 1, e->stream);
   if (st != hipSuccess) {
     *p = nullptr;
     return set_err(e, USAP_ENOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(st));
@@ -2452,7 +2630,7 @@ int build_by_pairs(usap_engine* e) {
   const size_t need = std::max(b1, b2);
   if (need > e->pair_tmp_bytes) {
     dfree(e, e->pair_tmp);
-    HIPCHK(hipMallocAsync(&e->pair_tmp, need, st));
+    if (int rc = dalloc_raw(e, &e->pair_tmp, need)) return rc;
     e->pair_tmp_bytes = need;
   }
   const unsigned g = std::min<int64_t>(blocks_for(E, 256), 16384);
@@ -2499,7 +2677,7 @@ int rebuild_M(usap_engine* e) {
                                  rocprim::plus<int64_t>(), st));
   if (bytes > e->scan_tmp_bytes) {
     dfree(e, e->scan_tmp);
-    HIPCHK(hipMallocAsync(&e->scan_tmp, bytes, st));
+    if (int rc = dalloc_raw(e, &e->scan_tmp, bytes)) return rc;
     e->scan_tmp_bytes = bytes;
   }
   bytes = e->scan_tmp_bytes;
@@ -2571,6 +2749,10 @@ int check_engine(usap_engine* e) {
 
 }  // namespace
 
+namespace {
+int refresh_host_graph(usap_engine* e);
+}  // namespace
+
 namespace usap {
 namespace {
 std::mutex g_err_mu;
@@ -2595,7 +2777,10 @@ const char* detached_error() {
 }
 
 std::vector<usap_phase>& engine_phase_log(usap_engine* e) { return e->phase_log; }
-const usap_graph* engine_host_graph(const usap_engine* e) { return &e->hg; }
+const usap_graph* engine_host_graph(usap_engine* e) {
+  if (refresh_host_graph(e) != USAP_OK) return nullptr;
+  return &e->hg;
+}
 const usap_params* engine_params(const usap_engine* e) { return &e->p; }
 int32_t engine_rebuilds(const usap_engine* e) { return e->rebuilds; }
 // Growth counters, read on the engine's own stream (never the legacy null
@@ -2703,6 +2888,9 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->status);
   dfree(e, e->sweep);
   dfree(e, e->prop_r);
+  dfree(e, e->prop_u);
+  dfree(e, e->aq_mark);
+  dfree(e, e->aq_acc);
   dfree(e, e->build_flags);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   dfree(e, e->tstamp);
@@ -2719,7 +2907,7 @@ int usap_destroy(usap_engine* e) {
     if (ev) (void)hipEventDestroy(ev);
   if (e->stream) {
     (void)hipStreamSynchronize(e->stream);
-    (void)hipStreamDestroy(e->stream);
+    if (e->own_stream) (void)hipStreamDestroy(e->stream);
   }
   delete e;
   return USAP_OK;
@@ -2759,16 +2947,18 @@ int check_graph(const usap_graph* g, int64_t* wtot) {
 // lambda[0..maxv] from one table shared by the process's engines (computed
 // again only when a graph needs more entries: batches of graphs would
 // otherwise spend most of their host time in log()).
-void lambda_prefix(int64_t maxv, int64_t* out) {
+void lambda_range(int64_t lo, int64_t hi, int64_t* out) {
   static std::mutex mu;
   static std::vector<int64_t> tab;
   std::lock_guard<std::mutex> lk(mu);
-  if ((int64_t)tab.size() <= maxv) {
-    tab.assign((size_t)maxv + 1, 0);
-    usap_lambda_table(maxv, tab.data());
+  if ((int64_t)tab.size() <= hi) {
+    tab.assign((size_t)hi + 1, 0);
+    usap_lambda_table(hi, tab.data());
   }
-  std::memcpy(out, tab.data(), sizeof(int64_t) * ((size_t)maxv + 1));
+  std::memcpy(out, tab.data() + lo, sizeof(int64_t) * (size_t)(hi - lo + 1));
 }
+
+void lambda_prefix(int64_t maxv, int64_t* out) { lambda_range(0, maxv, out); }
 
 #define TRY(x)          \
   do {                  \
@@ -2817,6 +3007,8 @@ int load_graph(usap_engine* e, const usap_graph* g, int64_t wtot) {
     TRY(dalloc(e, &e->in_w, E));
   }
   TRY(dalloc(e, &e->lam, wtot + 1));
+  e->lam_cap = wtot + 1;
+  e->host_stale = false;
   std::vector<int64_t> lam((size_t)wtot + 1);
   lambda_prefix(wtot, lam.data());
   std::vector<int32_t> src((size_t)E);
@@ -2852,6 +3044,206 @@ int load_graph(usap_engine* e, const usap_graph* g, int64_t wtot) {
   HIPCHK(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, st));
   // the host vectors above are released when this returns: wait for the copies
   HIPCHK(hipStreamSynchronize(st));
+  return USAP_OK;
+}
+
+// Global-memory neighbour tables of the nodes whose lists exceed the LDS
+// table (kTabMaxDeg), from the host copy of the offsets.
+int build_big_tables(usap_engine* e) {
+  const int32_t N = e->N;
+  std::vector<int64_t> big((size_t)N, -1);
+  int64_t big_total = 0;
+  for (int32_t x = 0; x < N; ++x) {
+    const int64_t k = (e->h_out_ptr[x + 1] - e->h_out_ptr[x]) + (e->h_in_ptr[x + 1] - e->h_in_ptr[x]);
+    if (k > kTabMaxDeg) {
+      int64_t cap = 1;
+      while (cap < 2 * k) cap <<= 1;
+      big[x] = big_total;
+      big_total += 3 * cap;
+    }
+  }
+  dfree(e, e->big_off);
+  dfree(e, e->big_tab);
+  TRY(dalloc(e, &e->big_off, N));
+  TRY(dalloc(e, &e->big_tab, big_total));
+  HIPCHK(hipMemcpyAsync(e->big_off, big.data(), sizeof(int64_t) * N, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));  // `big` is released on return
+  return USAP_OK;
+}
+
+// One side (CSR or CSC) of an append: the lists at ptr/idx/w grow by the n
+// edges whose node is key[i] and other end other[i] (device arrays).
+// owner (nullable): also rebuild the per-slot node array (the CSR's edge
+// sources).
+int append_side(usap_engine* e, int64_t n, const int32_t* key, const int32_t* other,
+                const int32_t* w, int64_t*& ptr, int32_t*& idx, int32_t*& wv, int32_t** owner) {
+  const int32_t N = e->N;
+  const int64_t E1 = e->E + n;
+  hipStream_t st = e->stream;
+  int32_t *cnt = nullptr, *kin = nullptr, *kout = nullptr, *vin = nullptr, *vout = nullptr;
+  int64_t *deg = nullptr, *nptr = nullptr;
+  int32_t *nidx = nullptr, *nw = nullptr, *nown = nullptr;
+  void* tmp = nullptr;
+  int rc = USAP_OK;
+  auto release = [&]() {
+    dfree(e, cnt);
+    dfree(e, kin);
+    dfree(e, kout);
+    dfree(e, vin);
+    dfree(e, vout);
+    dfree(e, deg);
+    dfree(e, tmp);
+  };
+  if ((rc = dalloc(e, &cnt, N)) || (rc = dalloc(e, &deg, (size_t)N + 1)) ||
+      (rc = dalloc(e, &nptr, (size_t)N + 1)) || (rc = dalloc(e, &kin, n)) ||
+      (rc = dalloc(e, &kout, n)) || (rc = dalloc(e, &vin, n)) || (rc = dalloc(e, &vout, n)) ||
+      (rc = dalloc(e, &nidx, E1)) || (e->weighted && (rc = dalloc(e, &nw, E1))) ||
+      (owner && (rc = dalloc(e, &nown, E1)))) {
+    release();
+    dfree(e, nptr);
+    dfree(e, nidx);
+    dfree(e, nw);
+    dfree(e, nown);
+    return rc;
+  }
+  int bits = 1;
+  while (((int64_t)1 << bits) < (int64_t)N) ++bits;
+  size_t b1 = 0, b2 = 0;
+  hipError_t he = rocprim::exclusive_scan(nullptr, b1, deg, nptr, (int64_t)0, (size_t)N + 1,
+                                          rocprim::plus<int64_t>(), st);
+  if (he == hipSuccess)
+    he = rocprim::radix_sort_pairs(nullptr, b2, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+  if (he == hipSuccess && !(rc = dalloc_raw(e, &tmp, std::max(b1, b2)))) {
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks_for(n, 256), 16384));
+    if ((he = hipMemsetAsync(cnt, 0, sizeof(int32_t) * N, st)) == hipSuccess) {
+      k_count_keys<<<g, 256, 0, st>>>(n, key, cnt);
+      k_new_deg<<<blocks_for((int64_t)N + 1, 256), 256, 0, st>>>(N, ptr, cnt, deg);
+      size_t b = std::max(b1, b2);
+      he = rocprim::exclusive_scan(tmp, b, deg, nptr, (int64_t)0, (size_t)N + 1,
+                                   rocprim::plus<int64_t>(), st);
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(kin, key, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st);
+    if (he == hipSuccess) {
+      k_iota<<<g, 256, 0, st>>>(n, vin);
+      size_t b = std::max(b1, b2);
+      he = rocprim::radix_sort_pairs(tmp, b, kin, kout, vin, vout, (size_t)n, 0, bits, st);
+    }
+    if (he == hipSuccess) {
+      k_move_lists<<<blocks_for(N, 256 / kWave), 256, 0, st>>>(N, ptr, nptr, idx, wv, nidx, nw, nown);
+      k_place_new<<<g, 256, 0, st>>>(n, kout, vout, ptr, other, w, nidx, nw, nown);
+      he = hipGetLastError();
+    }
+  }
+  release();
+  if (rc == USAP_OK && he != hipSuccess)
+    rc = set_err(e, USAP_EHIP, std::string("edge append: ") + hipGetErrorString(he));
+  if (rc != USAP_OK) {
+    dfree(e, nptr);
+    dfree(e, nidx);
+    dfree(e, nw);
+    dfree(e, nown);
+    return rc;
+  }
+  dfree(e, ptr);
+  dfree(e, idx);
+  dfree(e, wv);
+  ptr = nptr;
+  idx = nidx;
+  wv = nw;
+  if (owner) {
+    dfree(e, *owner);
+    *owner = nown;
+  }
+  return USAP_OK;
+}
+
+// The host copy of the graph from the device lists (after appends).
+int refresh_host_graph(usap_engine* e) {
+  if (!e->host_stale) return USAP_OK;
+  const int64_t E = e->E;
+  hipStream_t st = e->stream;
+  e->h_out_idx.resize((size_t)E);
+  e->h_in_idx.resize((size_t)E);
+  e->h_out_w.resize(e->weighted ? (size_t)E : 0);
+  e->h_in_w.resize(e->weighted ? (size_t)E : 0);
+  if (E > 0) {
+    HIPCHK(hipMemcpyAsync(e->h_out_idx.data(), e->out_idx, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(e->h_in_idx.data(), e->in_idx, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+    if (e->weighted) {
+      HIPCHK(hipMemcpyAsync(e->h_out_w.data(), e->out_w, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(e->h_in_w.data(), e->in_w, sizeof(int32_t) * E, hipMemcpyDeviceToHost, st));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  e->hg.E = E;
+  e->hg.out_ptr = e->h_out_ptr.data();
+  e->hg.in_ptr = e->h_in_ptr.data();
+  e->hg.out_idx = e->h_out_idx.data();
+  e->hg.in_idx = e->h_in_idx.data();
+  e->hg.out_w = e->weighted ? e->h_out_w.data() : nullptr;
+  e->hg.in_w = e->weighted ? e->h_in_w.data() : nullptr;
+  e->host_stale = false;
+  return USAP_OK;
+}
+
+// Appends n edges (host arrays, already checked) on the device.
+int append_edges(usap_engine* e, int64_t n, const int32_t* src, const int32_t* dst,
+                 const int32_t* w, int64_t wsum) {
+  hipStream_t st = e->stream;
+  const int64_t W1 = e->Wtot + wsum;
+  int32_t *ds = nullptr, *dd = nullptr, *dw = nullptr;
+  auto release = [&]() {
+    dfree(e, ds);
+    dfree(e, dd);
+    dfree(e, dw);
+  };
+  TRY(dalloc(e, &ds, n));
+  TRY(dalloc(e, &dd, n));
+  if (w) TRY(dalloc(e, &dw, n));
+  HIPCHK(hipMemcpyAsync(ds, src, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dd, dst, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  if (w) HIPCHK(hipMemcpyAsync(dw, w, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  int rc = append_side(e, n, ds, dd, dw, e->out_ptr, e->out_idx, e->out_w, &e->out_src);
+  if (rc == USAP_OK) rc = append_side(e, n, dd, ds, dw, e->in_ptr, e->in_idx, e->in_w, nullptr);
+  // the host arrays are the caller's: the copies above must finish first
+  const hipError_t he = hipStreamSynchronize(st);
+  release();
+  if (rc != USAP_OK) return rc;
+  if (he != hipSuccess) return set_err(e, USAP_EHIP, std::string("edge append: ") + hipGetErrorString(he));
+  e->E += n;
+  e->hg.E = e->E;
+  // lambda over [0, W1]: the new range only, the table grown geometrically
+  if (W1 + 1 > e->lam_cap) {
+    const int64_t cap = std::min<int64_t>((int64_t)1 << 31, std::max<int64_t>(W1 + 1, 2 * e->lam_cap));
+    int64_t* nl = nullptr;
+    TRY(dalloc(e, &nl, cap));
+    HIPCHK(hipMemcpyAsync(nl, e->lam, sizeof(int64_t) * (e->Wtot + 1), hipMemcpyDeviceToDevice, st));
+    std::vector<int64_t> add((size_t)(cap - 1 - e->Wtot));
+    lambda_range(e->Wtot + 1, cap - 1, add.data());
+    HIPCHK(hipMemcpyAsync(nl + e->Wtot + 1, add.data(), sizeof(int64_t) * add.size(),
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(e, e->lam);
+    e->lam = nl;
+    e->lam_cap = cap;
+  }
+  e->Wtot = W1;
+  // host offsets (big tables, driver) now; lists on demand
+  e->h_out_ptr.resize((size_t)e->N + 1);
+  e->h_in_ptr.resize((size_t)e->N + 1);
+  HIPCHK(hipMemcpyAsync(e->h_out_ptr.data(), e->out_ptr, sizeof(int64_t) * (e->N + 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(e->h_in_ptr.data(), e->in_ptr, sizeof(int64_t) * (e->N + 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  e->hg.out_ptr = e->h_out_ptr.data();
+  e->hg.in_ptr = e->h_in_ptr.data();
+  e->host_stale = true;
+  TRY(build_big_tables(e));
+  const int64_t ocap = std::min<int64_t>(kOvfCap, 2 * e->E + 4096);
+  if (ocap > e->M.ovf_cap) {  // the overflow buffer follows the edge count
+    dfree(e, e->M.ovf);
+    e->M.ovf_cap = (int32_t)ocap;
+    TRY(dalloc(e, &e->M.ovf, (size_t)4 * e->M.ovf_cap));
+  }
   return USAP_OK;
 }
 
@@ -2913,6 +3305,56 @@ int alloc_state(usap_engine* e) {
 extern "C" {
 
 int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
+  return usap::create_in(nullptr, g, p, out);
+}
+
+}  // extern "C"
+
+namespace usap {
+
+// A host worker's device context (see engine_internal.hpp).
+struct WorkerCtx {
+  int32_t device = 0;
+  hipStream_t stream = nullptr;
+  hipMemPool_t pool = nullptr;
+};
+
+int worker_ctx_create(int32_t device, WorkerCtx** out) {
+  if (!out) return USAP_EINVAL;
+  *out = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return USAP_EHIP;
+  WorkerCtx* w = new (std::nothrow) WorkerCtx();
+  if (!w) return USAP_ENOMEM;
+  w->device = device;
+  if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete w;
+    return USAP_EHIP;
+  }
+  hipMemPoolProps props{};
+  props.allocType = hipMemAllocationTypePinned;
+  props.location.type = hipMemLocationTypeDevice;
+  props.location.id = device;
+  if (hipMemPoolCreate(&w->pool, &props) != hipSuccess) {
+    (void)hipStreamDestroy(w->stream);
+    delete w;
+    return USAP_EHIP;
+  }
+  uint64_t keep = UINT64_MAX;  // freed blocks stay in the pool for the next graph
+  (void)hipMemPoolSetAttribute(w->pool, hipMemPoolAttrReleaseThreshold, &keep);
+  *out = w;
+  return USAP_OK;
+}
+
+void worker_ctx_destroy(WorkerCtx* w) {
+  if (!w) return;
+  (void)hipSetDevice(w->device);
+  (void)hipStreamSynchronize(w->stream);
+  (void)hipMemPoolDestroy(w->pool);
+  (void)hipStreamDestroy(w->stream);
+  delete w;
+}
+
+int create_in(WorkerCtx* ctx, const usap_graph* g, const usap_params* p, usap_engine** out) {
   if (!g || !out) return USAP_EINVAL;
   *out = nullptr;
   int64_t wtot = 0;
@@ -2936,12 +3378,22 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
     usap_destroy(e);
     return rc;
   }
-  prepare_pool(e->p.device);
-  hst = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
-  if (hst != hipSuccess) {
-    e->stream = nullptr;
-    usap_destroy(e);
-    return USAP_EHIP;
+  if (ctx) {
+    if (ctx->device != e->p.device) {
+      delete e;
+      return USAP_EINVAL;
+    }
+    e->stream = ctx->stream;
+    e->pool = ctx->pool;
+    e->own_stream = false;
+  } else {
+    prepare_pool(e->p.device);
+    hst = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (hst != hipSuccess) {
+      e->stream = nullptr;
+      usap_destroy(e);
+      return USAP_EHIP;
+    }
   }
   int rc = alloc_state(e);
   if (rc == USAP_OK) rc = load_graph(e, g, wtot);
@@ -2952,6 +3404,10 @@ int usap_create(const usap_graph* g, const usap_params* p, usap_engine** out) {
   *out = e;
   return USAP_OK;
 }
+
+}  // namespace usap
+
+extern "C" {
 
 int usap_set_graph(usap_engine* e, const usap_graph* g) {
   if (!e) return USAP_EINVAL;
@@ -2969,6 +3425,52 @@ int usap_set_graph(usap_engine* e, const usap_graph* g) {
   if (int rc = load_graph(e, g, wtot)) return rc;
   e->merge_B = -1;
   return e->has_labels ? rebuild_M(e) : USAP_OK;
+}
+
+int usap_append_edges(usap_engine* e, int64_t n, const int32_t* src, const int32_t* dst,
+                      const int32_t* w) {
+  if (!e || n < 0 || (n > 0 && (!src || !dst))) return USAP_EINVAL;
+  if (w != nullptr && !e->weighted)
+    return set_err(e, USAP_EINVAL, "usap_append_edges: weights for an unweighted graph");
+  if (n == 0) return USAP_OK;
+  int64_t wsum = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (src[i] < 0 || src[i] >= e->N || dst[i] < 0 || dst[i] >= e->N)
+      return set_err(e, USAP_EINVAL, "usap_append_edges: node id out of range");
+    const int64_t wi = w ? w[i] : 1;
+    if (wi < 0) return set_err(e, USAP_EINVAL, "usap_append_edges: negative weight");
+    wsum += wi;
+  }
+  if (e->Wtot + wsum >= (int64_t)1 << 31)
+    return set_err(e, USAP_EINVAL, "usap_append_edges: total weight reaches 2^31");
+  HIPCHK(hipSetDevice(e->p.device));
+  if (int rc = append_edges(e, n, src, dst, w, wsum)) return rc;
+  e->merge_B = -1;
+  return e->has_labels ? rebuild_M(e) : USAP_OK;
+}
+
+int usap_create_from_edges(int32_t N, int64_t E, const int32_t* src, const int32_t* dst,
+                           const int32_t* w, const usap_params* p, usap_engine** out) {
+  if (!out || N < 1 || E < 0 || (E > 0 && (!src || !dst))) return USAP_EINVAL;
+  *out = nullptr;
+  std::vector<int64_t> zero((size_t)N + 1, 0);
+  const int32_t dummy_w = 1;
+  usap_graph g{};
+  g.N = N;
+  g.E = 0;
+  g.out_ptr = zero.data();
+  g.in_ptr = zero.data();
+  g.out_w = w ? &dummy_w : nullptr;  // weighted iff weights are given
+  g.in_w = g.out_w;
+  usap_engine* e = nullptr;
+  if (int rc = usap_create(&g, p, &e)) return rc;
+  if (int rc = usap_append_edges(e, E, src, dst, w)) {
+    usap::set_detached_error(std::string("usap_create_from_edges: ") + e->err);
+    usap_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return USAP_OK;
 }
 
 int usap_set_labels(usap_engine* e, const int32_t* labels, int32_t B) {
@@ -3109,6 +3611,12 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   k_merge<<<B, nw * kWave, 0, e->stream>>>(a);
   HIPCHK(hipGetLastError());
   ev_rec(e, 1);
+  if (e->prof) {  // the reference algorithm's bytes: 2 (4 (B+1) + 8 nnz(M)) + 24 B
+    k_nnz_bytes<<<(unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, blocks_for(e->M.P, 256))),
+                  256, 0, e->stream>>>(e->M.kv, e->M.P, e->aq_acc + 2);
+    HIPCHK(hipGetLastError());
+    e->kp[USAP_K_MERGE].model_bytes += 8.0 * ((double)B + 1.0) + 24.0 * (double)B;
+  }
   e->merge_B = B;
   if (best_s || best_dS) {
     std::vector<long> h((size_t)B);
@@ -3160,7 +3668,7 @@ int usap_merge_apply(usap_engine* e, int32_t B_to_merge, int32_t* B_new) {
                                    e->sort_idx[1], (size_t)B, 0, 64, st));
   if (bytes > e->sort_tmp_bytes) {
     dfree(e, e->sort_tmp);
-    HIPCHK(hipMallocAsync(&e->sort_tmp, bytes, st));
+    if (int rc = dalloc_raw(e, &e->sort_tmp, bytes)) return rc;
     e->sort_tmp_bytes = bytes;
   }
   k_sort_init<<<blocks_for(B, 256), 256, 0, st>>>(B, e->best_lo, e->sort_key[0], e->sort_idx[0]);
@@ -3300,6 +3808,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   a.prop_inf = e->prop_inf;
   a.prop_acc = e->prop_acc;
   a.prop_r = e->prop_r;
+  a.prop_u = e->prof ? e->prop_u : nullptr;
   a.part = e->part;
   a.status = e->status;
   a.sweep = e->sweep;
@@ -3352,6 +3861,22 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   if (e->prof && (int64_t)grid > G)
     return set_err(e, USAP_ESTATE, "profiled batch larger than its stamp slot");
   k_nodal<<<grid, kWave * kNodalWaves, 0, st>>>(a);
+  if (e->prof) {  // the reference algorithm's bytes for this batch (M not yet changed)
+    AqArgs q;
+    q.g = a.g;
+    q.M = a.M;
+    q.beg = beg;
+    q.end = end;
+    q.prop_r = e->prop_r;
+    q.prop_s = e->prop_s;
+    q.prop_u = e->prop_u;
+    q.prop_acc = e->prop_acc;
+    q.mark = e->aq_mark;
+    q.stamp = e->aq_stamp++;
+    q.sweep = e->sweep;
+    q.acc = e->aq_acc;
+    k_aq<<<grid, kWave * kNodalWaves, 0, st>>>(q);
+  }
   k_apply<<<grid + 1, kWave * kNodalWaves, 0, st>>>(b);  // block 0: the update step
   if (grow) k_grow<<<kGrowBlocks, 256, 0, st>>>(ga);
   HIPCHK(hipGetLastError());
@@ -3597,10 +4122,26 @@ int usap_nodal_detail(usap_engine* e, int32_t* s, int64_t* hi, uint64_t* lo, int
 
 int usap_profile(usap_engine* e, int32_t enable, usap_kernel_prof* out) {
   if (!e) return USAP_EINVAL;
-  if (out)
+  if (out) {
     for (int k = 0; k < USAP_NKERNELS; ++k) out[k] = e->kp[k];
+    if (e->aq_acc && e->prof) {
+      unsigned long long acc[3];
+      HIPCHK(hipMemcpyAsync(acc, e->aq_acc, sizeof(acc), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      out[USAP_K_NODAL].model_bytes += (double)acc[0] + (double)acc[1];
+      out[USAP_K_MERGE].model_bytes += (double)acc[2];
+    }
+  }
   if (enable == 1) {
     for (int k = 0; k < USAP_NKERNELS; ++k) e->kp[k] = usap_kernel_prof{};
+    if (!e->prop_u) {
+      if (int rc = dalloc(e, &e->prop_u, e->N)) return rc;
+      if (int rc = dalloc(e, &e->aq_mark, e->N)) return rc;
+      if (int rc = dalloc(e, &e->aq_acc, 3)) return rc;
+      HIPCHK(hipMemsetAsync(e->aq_mark, 0xff, sizeof(int32_t) * e->N, e->stream));
+      e->aq_stamp = 0;
+    }
+    HIPCHK(hipMemsetAsync(e->aq_acc, 0, 3 * sizeof(unsigned long long), e->stream));
     e->prof = true;
   } else if (enable == 0) {
     e->prof = false;

@@ -7,8 +7,9 @@
 #include "usap.h"
 
 namespace usap {
-// Host copy of the graph the engine was created with.
-const usap_graph* engine_host_graph(const usap_engine* e);
+// Host copy of the engine's graph (refreshed from the device after edge
+// appends); null if that copy failed.
+const usap_graph* engine_host_graph(usap_engine* e);
 // Per-phase records of the running / last partition (usap_partition_log).
 std::vector<usap_phase>& engine_phase_log(usap_engine* e);
 // usap_nodal_sweep; with S null the sweep starts from the entropy of the
@@ -26,6 +27,15 @@ void engine_set_error(usap_engine* e, const char* msg);
 // Error of a call that has no engine handle to hold it (batch, serve): the
 // first one recorded since the last clear; usap_last_error(NULL) returns it.
 void set_detached_error(const std::string& msg);
+// A host worker's device context: one HIP stream and one stream-ordered
+// memory pool that the engines the worker creates one after another share
+// (batched and served graphs), so that no stream or pool is created or
+// destroyed per graph and no freed block moves between workers' streams.
+struct WorkerCtx;
+int worker_ctx_create(int32_t device, WorkerCtx** out);
+void worker_ctx_destroy(WorkerCtx* w);
+// usap_create on a worker context (null: own stream, the device's pool).
+int create_in(WorkerCtx* ctx, const usap_graph* g, const usap_params* p, usap_engine** out);
 void clear_detached_error();
 const char* detached_error();
 }  // namespace usap

@@ -4,13 +4,16 @@
 // No counterpart in the reference (it lists streaming as future work,
 // README.md:47-49).  Graph i arrives at arrival_s; its edges come in `stages`
 // increments, interval_s apart: stage k holds the first ceil(E*k/stages)
-// edges of its delivery order.  Stage 1 is a full partition()
-// (usap_partition); stage k > 1 replaces the engine's edges (usap_set_graph,
-// labels kept) and resumes the nodal chain from the previous stage's
-// partition (usap_resume); optionally the last stage partitions the whole
-// graph afresh, since the resumed chain keeps the block count stage 1 chose
-// on a tenth of the edges.  A graph's engine -- labels, M, degrees -- stays
-// resident on the device from its first stage to its last.
+// edges of its delivery order.  Stage 1 builds the engine from its edges on
+// the device (usap_create_from_edges) and runs a full partition()
+// (usap_partition); stage k > 1 appends its increment on the device
+// (usap_append_edges, labels kept) and resumes the nodal chain from the
+// previous stage's partition (usap_resume).  The resumed chain keeps the
+// block count of the last full partition, and merges cannot add blocks, so
+// a stage runs a full partition again once the graph has grown by
+// `repartition_growth` since the last one, and, optionally, at the last
+// stage.  A graph's engine -- lists, labels, M, degrees -- stays resident on
+// the device from its first stage to its last.
 //
 // Continuous batching: `workers` host threads take whatever stage is ready
 // (its increment has arrived and the graph's previous stage is done), oldest
@@ -44,6 +47,7 @@ struct GraphState {
   bool running = false;
   bool done = false;
   int64_t bytes = 0;       // footprint charged to the budget while resident
+  int64_t E_full = 0;      // edges at the graph's last full partition
 };
 
 struct Server {
@@ -116,44 +120,38 @@ struct Server {
     return t;
   }
 
-  // Runs stage k of graph g (outside mu).
+  // Runs stage k of graph g (outside mu).  Only the stage's increment goes
+  // to the device: stage 1 builds the engine's lists from its edges, later
+  // stages append theirs.
   int run_stage(int32_t g, int32_t k, usap_serve_stage& rec) {
     const usap_serve_graph& G = graphs[g];
     const int64_t Ek = (int64_t)std::ceil((double)G.E * k / opt.stages);
-    std::vector<int64_t> out_ptr((size_t)G.N + 1), in_ptr((size_t)G.N + 1);
-    std::vector<int32_t> out_idx((size_t)std::max<int64_t>(Ek, 1)),
-        in_idx((size_t)std::max<int64_t>(Ek, 1)), out_w, in_w;
-    if (G.w) {
-      out_w.resize(out_idx.size());
-      in_w.resize(in_idx.size());
-    }
-    int rc = usap_build_csr(G.N, Ek, G.src, G.dst, G.w, out_ptr.data(), out_idx.data(),
-                            G.w ? out_w.data() : nullptr, in_ptr.data(), in_idx.data(),
-                            G.w ? in_w.data() : nullptr);
-    if (rc) return rc;
-    usap_graph ug{};
-    ug.N = G.N;
-    ug.E = Ek;
-    ug.out_ptr = out_ptr.data();
-    ug.out_idx = out_idx.data();
-    ug.in_ptr = in_ptr.data();
-    ug.in_idx = in_idx.data();
-    ug.out_w = G.w ? out_w.data() : nullptr;
-    ug.in_w = G.w ? in_w.data() : nullptr;
+    const int64_t Ep = (int64_t)std::ceil((double)G.E * (k - 1) / opt.stages);
     usap_stats st;
     std::memset(&st, 0, sizeof(st));
     usap_engine*& e = gs[g].eng;
     int32_t* lab = labels + label_off[g];
+    int rc = USAP_OK;
     if (k == 1) {
-      rc = usap_create(&ug, &G.params, &e);
+      rc = usap_create_from_edges(G.N, Ek, G.src, G.dst, G.w, &G.params, &e);
       if (rc == USAP_OK) rc = usap_partition(e, lab, &st);
-    } else if (k == opt.stages && opt.final_partition) {
-      rc = usap_set_graph(e, &ug);
-      if (rc == USAP_OK) rc = usap_partition(e, lab, &st);
+      gs[g].E_full = Ek;
     } else {
-      rc = usap_set_graph(e, &ug);
-      // each stage draws from its own range of nodal streams
-      if (rc == USAP_OK) rc = usap_resume(e, (uint64_t)k << 40, lab, &st);
+      rc = usap_append_edges(e, Ek - Ep, G.src + Ep, G.dst + Ep, G.w ? G.w + Ep : nullptr);
+      // a full partition (merge phases and the search over B) at the last
+      // stage, or once the graph has grown by the given factor since the
+      // last one; else the nodal chain resumes at the current B
+      const bool full = (k == opt.stages && opt.final_partition) ||
+                        (opt.repartition_growth > 1.0 &&
+                         (double)Ek >= opt.repartition_growth * (double)gs[g].E_full);
+      if (rc == USAP_OK && full) {
+        rc = usap_partition(e, lab, &st);
+        gs[g].E_full = Ek;
+        rec.full = 1;
+      } else if (rc == USAP_OK) {
+        // each stage draws from its own range of nodal streams
+        rc = usap_resume(e, (uint64_t)k << 40, lab, &st);
+      }
     }
     rec.S = st.S;
     rec.B = st.B;

@@ -29,13 +29,31 @@ def fx(hi, lo):
 class Engine:
     """One graph resident on the device: labels, M (block rows holding both orientations), block degrees."""
 
-    def __init__(self, graph, **params):
+    def __init__(self, graph, edges=None, **params):
+        """edges = (src, dst, w or None) in delivery order: the CSR/CSC are
+        built on the device from that list (usap_create_from_edges) instead
+        of uploaded from graph (which then only supplies N and the truth)."""
         self.graph = graph
         self.params = default_params(**params)
-        self._gs = graph.as_struct()
         h = C.c_void_p()
-        check(lib().usap_create(C.byref(self._gs), C.byref(self.params), C.byref(h)))
+        if edges is not None:
+            src, dst, w = edges
+            check(lib().usap_create_from_edges(
+                graph.N, len(src), ptr(np.ascontiguousarray(src, np.int32)),
+                ptr(np.ascontiguousarray(dst, np.int32)),
+                None if w is None else ptr(np.ascontiguousarray(w, np.int32)),
+                C.byref(self.params), C.byref(h)))
+        else:
+            self._gs = graph.as_struct()
+            check(lib().usap_create(C.byref(self._gs), C.byref(self.params), C.byref(h)))
         self.h = h.value
+
+    def append_edges(self, src, dst, w=None):
+        """Append edges after the current ones (usap_append_edges)."""
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        wp = None if w is None else ptr(np.ascontiguousarray(w, np.int32))
+        self._ck(lib().usap_append_edges(self.h, len(src), ptr(src), ptr(dst), wp))
 
     def close(self):
         if getattr(self, "h", None):
@@ -174,10 +192,13 @@ class Engine:
 
     def profile(self, enable=-1):
         """enable=1 resets and starts kernel timing, 0 stops, -1 queries.
-        Returns {kernel: {launches, ms, bytes}} accumulated so far."""
+        Returns {kernel: {launches, ms, bytes, model_bytes}} accumulated so
+        far (model_bytes: the reference algorithm's compulsory bytes for the
+        same work, nodal and merge kernels)."""
         arr = (UsapKernelProf * len(KERNELS))()
         self._ck(lib().usap_profile(self.h, enable, arr))
-        return {k: {"launches": a.launches, "ms": a.ms, "bytes": a.bytes}
+        return {k: {"launches": a.launches, "ms": a.ms, "bytes": a.bytes,
+                    "model_bytes": a.model_bytes}
                 for k, a in zip(KERNELS, arr)}
 
 

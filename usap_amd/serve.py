@@ -53,9 +53,11 @@ def poisson_arrivals(n, rate, seed=5):
 
 
 def serve(graphs, arrivals, params, stages=10, interval_s=0.0, workers=4, hbm_budget=0,
-          final_partition=False):
+          final_partition=False, repartition_growth=0.0):
     """Serves the graphs (params: dict for all or one per graph).  Returns
-    (final labels per graph, stage records [graph][stage], stats dict)."""
+    (final labels per graph, stage records [graph][stage], stats dict).
+    repartition_growth > 1: a stage whose edges reach that multiple of the
+    edges at the graph's last full partition partitions afresh."""
     n = len(graphs)
     arr = (UsapServeGraph * max(n, 1))()
     for i, g in enumerate(graphs):
@@ -67,7 +69,8 @@ def serve(graphs, arrivals, params, stages=10, interval_s=0.0, workers=4, hbm_bu
         arr[i].w = None if g.w is None else g.w.ctypes.data
         arr[i].arrival_s = float(arrivals[i])
         arr[i].params = p if isinstance(p, UsapParams) else default_params(**p)
-    opt = UsapServeOpts(stages, float(interval_s), workers, int(hbm_budget), int(final_partition))
+    opt = UsapServeOpts(stages, float(interval_s), workers, int(hbm_budget), int(final_partition),
+                        float(repartition_growth))
     off = np.zeros(n + 1, np.int64)
     off[1:] = np.cumsum([g.N for g in graphs])
     labels = np.zeros(max(int(off[-1]), 1), np.int32)
