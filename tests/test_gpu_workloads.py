@@ -1,0 +1,71 @@
+"""The batched and served workloads' own graphs (not only small stand-ins):
+graphs of the batched workload (256 DCSBMs, N log-uniform in [500, 50000])
+partitioned with several in flight equal the oracle's partitions, and the
+served workload's largest graph (N near 200000) gets the same stages served
+alone and in a mix."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(*a):
+    print(*a, flush=True)
+
+
+def test_batched_workload_graphs_against_oracle(usap, orc):
+    """24 graphs of the batched workload spread over its size range, the
+    largest included, partitioned 8 in flight (reference defaults: SCC init,
+    ref dS, batches of N/4): each equals its alone partition, and three of
+    them (smallest, middle, largest) equal the oracle's -- labels, B, S,
+    sweep and batch counts."""
+    graphs, specs = usap.batch_graphs(n=256, seed=4)
+    order = np.argsort([g.N for g in graphs])
+    pick = [int(order[k * (len(order) - 1) // 23]) for k in range(24)]
+    gs = [graphs[i] for i in pick]
+    params = [dict(seed=specs[i]["seed"]) for i in pick]
+    _flush(f"batched: {len(gs)} graphs, N {gs[0].N}..{gs[-1].N}")
+    labs, sts, bs = usap.partition_batch(gs, params, workers=8)
+    assert bs["failed"] == 0
+    for k in (0, 12, 23):
+        g, p = gs[k], params[k]
+        _flush(f"oracle: graph N={g.N} E={g.E}")
+        o = orc.Oracle(g, threads=16, **p)
+        o.set_labels(*o.scc())
+        B, olab, ost = o.partition()
+        o.close()
+        assert sts[k]["B"] == B and sts[k]["S"] == ost["S"], k
+        assert (sts[k]["sweeps"], sts[k]["batches"]) == (ost["sweeps"], ost["batches"])
+        np.testing.assert_array_equal(labs[k], olab)
+    for k in range(24):
+        with usap.Engine(gs[k], **params[k]) as eng:
+            lab, st = eng.partition()
+        np.testing.assert_array_equal(lab, labs[k])
+        assert st["S"] == sts[k]["S"]
+
+
+def test_served_workload_largest_graph_alone_and_in_mix(usap):
+    """The served workload's largest graph and three others arriving at once,
+    ten increments each, 4 stages in flight, repartition on doubling: the
+    largest graph's per-stage MDL and final labels equal those it gets served
+    alone."""
+    from usap_amd import serve as sv
+    graphs = sv.served_graphs(n=24, seed=5)
+    order = np.argsort([g.N for g in graphs])
+    pick = [int(order[-1]), int(order[0]), int(order[8]), int(order[16])]
+    gs = [graphs[i] for i in pick]
+    _flush(f"served: N {[g.N for g in gs]}")
+    params = [dict(seed=g.seed) for g in gs]
+    labs, per, st = sv.serve(gs, np.zeros(len(gs)), params, stages=10, workers=4,
+                             repartition_growth=2.0)
+    assert st["failed"] == 0
+    lab1, per1, _ = sv.serve([gs[0]], [0.0], [params[0]], stages=10, workers=1,
+                             repartition_growth=2.0)
+    np.testing.assert_array_equal(labs[0], lab1[0])
+    assert [r["S"] for r in per[0]] == [r["S"] for r in per1[0]]
+    E = [int(np.ceil(gs[0].E * k / 10)) for k in range(1, 11)]
+    want, last = [], None
+    for k, Ek in enumerate(E, 1):
+        want.append(int(k == 1 or Ek >= 2 * last))
+        last = Ek if want[-1] else last
+    assert [r["full"] for r in per[0]] == want

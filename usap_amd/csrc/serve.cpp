@@ -136,6 +136,7 @@ struct Server {
       rc = usap_create_from_edges(G.N, Ek, G.src, G.dst, G.w, &G.params, &e);
       if (rc == USAP_OK) rc = usap_partition(e, lab, &st);
       gs[g].E_full = Ek;
+      rec.full = 1;
     } else {
       rc = usap_append_edges(e, Ek - Ep, G.src + Ep, G.dst + Ep, G.w ? G.w + Ep : nullptr);
       // a full partition (merge phases and the search over B) at the last
