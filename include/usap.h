@@ -156,6 +156,18 @@ int usap_get_M_coo(usap_engine* e, int64_t cap, int32_t* rows, int32_t* cols, in
  * data_fx_hi/lo (nullable): exact data_S as int128 words. */
 int usap_entropy(usap_engine* e, double* S, int64_t* data_fx_hi, uint64_t* data_fx_lo);
 
+/* usap_entropy without any allocation or host wait.  No counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  Launches on the engine's stream and writes the MDL
+ * of the current state to device memory the caller owns: S_dev (one double,
+ * nullable) and fx_dev (the exact data term as int128 words hi, lo;
+ * nullable); the values equal usap_entropy's once the stream has reached
+ * them.  Partial sums live in a 64 KiB static device array (4096 partials,
+ * the most the grid-strided scan uses), one per GPU, shared by every call on
+ * that GPU and holding the last call's contents: calls that overlap on one
+ * GPU race.  USAP_EINVAL if both outputs are NULL. */
+int usap_entropy_noalloc(usap_engine* e, double* S_dev, int64_t* fx_dev);
+
 /* K merge proposals per block and the best per block.
  * ≙ block_merge task (graph_parallel.hpp:349-368) over
  *   _propose_new_partition_block (graph_parallel.hpp:761-988).

@@ -80,6 +80,9 @@ constexpr int kScr = 3 * kTab;           // per-wavefront LDS scratch (ints) for
 constexpr int kDegLds = 1024;            // k_degrees privatises its counters for B <= this
 constexpr int kSmallB = 128;             // k_insert_small: B*B counts in LDS
 constexpr int kEntropyBlocks = 4096;  // k_entropy grid at most
+// Partials of the allocation-free MDL (usap_entropy_noalloc): 2 words per
+// k_entropy workgroup, 64 KiB per GPU.
+static __device__ __align__(256) long scratch[2 * kEntropyBlocks];
 constexpr int kNumSnap = 4;
 
 enum : int32_t {
@@ -3557,12 +3560,76 @@ namespace {
 // k_entropy over the pool in use: about four slots per thread, at most
 // kEntropyBlocks workgroups (a scan of 1 GB pools at B = 250k is bound by the
 // loads in flight).  Returns the grid (the number of partials).
-int launch_entropy(usap_engine* e) {
+int launch_entropy(usap_engine* e, long* part = nullptr) {
   const int64_t work = std::max<int64_t>(e->M.P, e->B);
   const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(kEntropyBlocks, blocks_for(work, 1024)));
   k_entropy<<<grid, 256, 0, e->stream>>>(e->M.kv, e->M.gst, e->d_out, e->d_in, e->B, e->lam,
-                                         e->M.P, e->ent_part);
+                                         e->M.P, part ? part : e->ent_part);
   return grid;
+}
+
+// ---- allocation-free MDL -----------------------------------------------------
+// usap_entropy_noalloc keeps k_entropy's per-workgroup partials in this
+// file-scope array instead of the engine's buffer: kEntropyBlocks partials of
+// 16 bytes (64 KiB), one copy per GPU the library runs on, held until the
+// process ends.  Host code never passes the symbol itself: each device's copy
+// is reached through the address hipGetSymbolAddress returns for it on that
+// device.
+constexpr int kScratchParts = kEntropyBlocks;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+long* g_scratch_ptr[kMaxDevices] = {};
+
+// The MDL from k_entropy's nparts partials, into the caller's device memory
+// (S: model + data term; fx: the data term's int128 words, hi then lo): the
+// same integer sum and IEEE operations as usap_entropy on the host.
+__global__ __launch_bounds__(256) void k_entropy_out(const long* __restrict__ part, int32_t nparts,
+                                                     double model, double* S, int64_t* fx) {
+  __shared__ long s_hi[4];
+  __shared__ unsigned long s_lo[4];
+  i128 t = 0;
+  for (int32_t i = threadIdx.x; i < nparts; i += blockDim.x)
+    t += (i128)(((u128)(unsigned long)part[2 * i] << 64) | (u128)(unsigned long)part[2 * i + 1]);
+  t = wave_sum128(t);
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    s_hi[w] = (long)(t >> 64);
+    s_lo[w] = (unsigned long)(u128)t;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    i128 tot = 0;
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i)
+      tot += (i128)(((u128)(unsigned long)s_hi[i] << 64) | (u128)s_lo[i]);
+    const i128 data = -tot;
+    if (S) *S = model + fx_to_double_dev(data);
+    if (fx) {
+      fx[0] = (int64_t)(data >> 64);
+      fx[1] = (int64_t)(uint64_t)(u128)data;
+    }
+  }
+}
+
+// This device's copy of `scratch`: looked up once per device, with the
+// device current, on the first allocation-free call there.
+int scratch_on(usap_engine* e, long** out) {
+  const int dev = e->p.device;
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_err(e, USAP_EINVAL, "usap_entropy_noalloc: device ordinal beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (st != hipSuccess || !p)
+      return set_err(e, USAP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(st));
+    g_scratch_ptr[dev] = static_cast<long*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return USAP_OK;
 }
 }  // namespace
 
@@ -3586,6 +3653,22 @@ int usap_entropy(usap_engine* e, double* S, int64_t* hi, uint64_t* lo) {
   if (hi) *hi = (int64_t)(data >> 64);
   if (lo) *lo = (uint64_t)(u128)data;
   if (S) *S = usap_model_S(e->N, e->E, e->B) + fx_to_double(data);
+  return USAP_OK;
+}
+
+int usap_entropy_noalloc(usap_engine* e, double* S_dev, int64_t* fx_dev) {
+  if (int rc = check_engine(e)) return rc;
+  if (!S_dev && !fx_dev) return USAP_EINVAL;
+  long* part = nullptr;
+  if (int rc = scratch_on(e, &part)) return rc;
+  // k_entropy is grid-strided over the pool with at most kScratchParts
+  // workgroups, so every engine state fits the scratch
+  const int64_t need = std::min<int64_t>(kEntropyBlocks, blocks_for(std::max<int64_t>(e->M.P, e->B), 1024));
+  if (need > kScratchParts)
+    return set_err(e, USAP_EINVAL, "usap_entropy_noalloc: more partials than the 64 KiB scratch holds");
+  const int grid = launch_entropy(e, part);
+  k_entropy_out<<<1, 256, 0, e->stream>>>(part, grid, usap_model_S(e->N, e->E, e->B), S_dev, fx_dev);
+  HIPCHK(hipGetLastError());
   return USAP_OK;
 }
 
