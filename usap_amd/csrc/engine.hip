@@ -1743,7 +1743,10 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
 // 8-byte read-modify-writes per list entry (16 k_n).  A block counts once per
 // batch: the first wavefront to raise its mark to the batch's stamp counts
 // the non-zeros of its block row (o > 0: row b, i > 0: column b).  One
-// wavefront per batch node; acc[0] += per-node bytes, acc[1] += block bytes.
+// wavefront per batch node; each workgroup adds its per-node bytes to
+// acc[2 j] and its block bytes to acc[2 j + 1], j = workgroup % kAqSlots
+// (spread over slots: one address would serialise the whole grid's adds).
+constexpr int kAqSlots = 64;
 struct AqArgs {
   GraphView g;
   MView M;
@@ -1762,13 +1765,14 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_aq(AqArgs a) {
   const int lane = lane_id();
   const int64_t item = (int64_t)blockIdx.x * kNodalWaves + threadIdx.x / kWave;
   const int64_t n64 = (int64_t)a.beg + item;
-  if (n64 >= a.end || a.sweep->stop_n) return;  // uniform per wavefront
-  const int32_t n = (int32_t)n64;
-  const int64_t kc = (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]);
+  const bool live = n64 < a.end && a.sweep->stop_n == 0;  // uniform per wavefront
+  const int32_t n = live ? (int32_t)n64 : a.beg;
+  const int64_t kc =
+      live ? (a.g.out_ptr[n + 1] - a.g.out_ptr[n]) + (a.g.in_ptr[n + 1] - a.g.in_ptr[n]) : 0;
   int32_t b = -1;
-  if (lane == 0) b = a.prop_r[item];
-  if (lane == 1) b = a.prop_s[item];
-  if (lane == 2) b = a.prop_u[item];
+  if (live && lane == 0) b = a.prop_r[item];
+  if (live && lane == 1) b = a.prop_s[item];
+  if (live && lane == 2) b = a.prop_u[item];
   const bool first = b >= 0 && atomicMax(&a.mark[b], a.stamp) < a.stamp;
   unsigned long long rows = __ballot(first);
   unsigned long long blk = 0;
@@ -1785,17 +1789,34 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_aq(AqArgs a) {
     }
     blk += 16 + 8 * (unsigned long long)wave_sum(nz) + 12;
   }
+  __shared__ unsigned long long s_node[kNodalWaves], s_blk[kNodalWaves];
+  const int w = threadIdx.x / kWave;
   if (lane == 0) {
-    unsigned long long node = 16 + 4 + 8 * (unsigned long long)kc + 12;
-    if (a.g.out_w) node += 4 * (unsigned long long)kc;
-    if (a.prop_acc[item]) node += 4 + 16 * (unsigned long long)kc;
-    atomicAdd(&a.acc[0], node);
-    if (blk) atomicAdd(&a.acc[1], blk);
+    unsigned long long node = 0;
+    if (live) {
+      node = 16 + 4 + 8 * (unsigned long long)kc + 12;
+      if (a.g.out_w) node += 4 * (unsigned long long)kc;
+      if (a.prop_acc[item]) node += 4 + 16 * (unsigned long long)kc;
+    }
+    s_node[w] = node;
+    s_blk[w] = blk;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long node = 0, bsum = 0;
+    for (int i = 0; i < kNodalWaves; ++i) {
+      node += s_node[i];
+      bsum += s_blk[i];
+    }
+    const int j = (int)(blockIdx.x % kAqSlots);
+    atomicAdd(&a.acc[2 * j], node);
+    if (bsum) atomicAdd(&a.acc[2 * j + 1], bsum);
   }
 }
 
 // Profiling only: 16 bytes per non-zero of M (its entry read from the row
-// and from the column structure of graph_parallel.hpp:761-988) into acc[0].
+// and from the column structure of graph_parallel.hpp:761-988), spread over
+// acc[0 .. kAqSlots).
 __global__ __launch_bounds__(256) void k_nnz_bytes(const Slot* __restrict__ kv, int64_t P,
                                                    unsigned long long* acc) {
   long c = 0;
@@ -1803,7 +1824,9 @@ __global__ __launch_bounds__(256) void k_nnz_bytes(const Slot* __restrict__ kv, 
        j += (int64_t)gridDim.x * blockDim.x)
     c += kv[j].o > 0 ? 1 : 0;
   c = wave_sum(c);
-  if (lane_id() == 0 && c) atomicAdd(acc, 16ull * (unsigned long long)c);
+  if (lane_id() == 0 && c)
+    atomicAdd(&acc[(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kAqSlots],
+              16ull * (unsigned long long)c);
 }
 
 // M build when every block holds one node (the identity start, B == N): block
@@ -3696,7 +3719,7 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   ev_rec(e, 1);
   if (e->prof) {  // the reference algorithm's bytes: 2 (4 (B+1) + 8 nnz(M)) + 24 B
     k_nnz_bytes<<<(unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, blocks_for(e->M.P, 256))),
-                  256, 0, e->stream>>>(e->M.kv, e->M.P, e->aq_acc + 2);
+                  256, 0, e->stream>>>(e->M.kv, e->M.P, e->aq_acc + 2 * kAqSlots);
     HIPCHK(hipGetLastError());
     e->kp[USAP_K_MERGE].model_bytes += 8.0 * ((double)B + 1.0) + 24.0 * (double)B;
   }
@@ -4208,11 +4231,11 @@ int usap_profile(usap_engine* e, int32_t enable, usap_kernel_prof* out) {
   if (out) {
     for (int k = 0; k < USAP_NKERNELS; ++k) out[k] = e->kp[k];
     if (e->aq_acc && e->prof) {
-      unsigned long long acc[3];
+      unsigned long long acc[3 * kAqSlots];
       HIPCHK(hipMemcpyAsync(acc, e->aq_acc, sizeof(acc), hipMemcpyDeviceToHost, e->stream));
       HIPCHK(hipStreamSynchronize(e->stream));
-      out[USAP_K_NODAL].model_bytes += (double)acc[0] + (double)acc[1];
-      out[USAP_K_MERGE].model_bytes += (double)acc[2];
+      for (int j = 0; j < 2 * kAqSlots; ++j) out[USAP_K_NODAL].model_bytes += (double)acc[j];
+      for (int j = 0; j < kAqSlots; ++j) out[USAP_K_MERGE].model_bytes += (double)acc[2 * kAqSlots + j];
     }
   }
   if (enable == 1) {
@@ -4220,11 +4243,11 @@ int usap_profile(usap_engine* e, int32_t enable, usap_kernel_prof* out) {
     if (!e->prop_u) {
       if (int rc = dalloc(e, &e->prop_u, e->N)) return rc;
       if (int rc = dalloc(e, &e->aq_mark, e->N)) return rc;
-      if (int rc = dalloc(e, &e->aq_acc, 3)) return rc;
+      if (int rc = dalloc(e, &e->aq_acc, 3 * kAqSlots)) return rc;
       HIPCHK(hipMemsetAsync(e->aq_mark, 0xff, sizeof(int32_t) * e->N, e->stream));
       e->aq_stamp = 0;
     }
-    HIPCHK(hipMemsetAsync(e->aq_acc, 0, 3 * sizeof(unsigned long long), e->stream));
+    HIPCHK(hipMemsetAsync(e->aq_acc, 0, 3 * kAqSlots * sizeof(unsigned long long), e->stream));
     e->prof = true;
   } else if (enable == 0) {
     e->prof = false;
