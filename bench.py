@@ -429,6 +429,9 @@ def main():
     el = time.perf_counter() - t0
     props = sum(s["nodal_proposals"] for s in stats)
     value = props / el
+    for i, p in enumerate(eng.partition_log()):  # the last timed partition, per phase
+        log(f"[bench] timed phase {i}: B={p['B']} sweeps={p['sweeps']} "
+            f"ms merge={p['ms_merge']:.1f} nodal={p['ms_nodal']:.1f}")
     # kernel times and byte counts from one more partition, outside the timed
     # region (the byte counts run kernels of their own)
     eng.profile(1)
@@ -438,6 +441,12 @@ def main():
     prof = eng.profile(0)
     plog = eng.partition_log()
     assert pst["S"] == stats[-1]["S"] and pst["batches"] == stats[-1]["batches"]
+    for i, p in enumerate(plog):
+        log(f"[bench] phase {i}: B={p['B']} sweeps={p['sweeps']} proposals={p['proposals']} "
+            f"moves={p['moves']} ms merge={p['ms_merge']:.1f} nodal={p['ms_nodal']:.1f} "
+            f"k_nodal={p['k_nodal_ms']:.1f} k_apply={p['k_apply_ms']:.1f} "
+            f"k_grow={p['k_grow_ms']:.1f} k_merge={p['k_merge_ms']:.1f} "
+            f"build={p['k_build_ms']:.1f} rebuilds={p['rebuilds']}")
 
     kp = prof["nodal"]
     secs = kp["ms"] / 1e3
