@@ -362,7 +362,7 @@ int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts*
 
 /* Picks the number of graphs in flight: partitions a sample of `sample`
  * graphs spread over the size range (0 = 24) with 1, 2, 4, 8, ... workers (at
- * most max_workers, 0 = 16) while graphs/s improves by 2% or more.  tried /
+ * most max_workers, 0 = 16) and takes the count with the best graphs/s.  tried /
  * rates (nullable, >= 16 entries) receive each trial's workers and graphs/s. */
 int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* params,
                         int32_t max_workers, int32_t sample, int32_t* workers, int32_t* tried,

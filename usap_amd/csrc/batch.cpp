@@ -156,8 +156,9 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
   std::vector<int64_t> off(n + 1, 0);
   for (int32_t i = 0; i < n; ++i) off[i + 1] = off[i] + graphs[i].N;
   std::vector<int32_t> scratch((size_t)off[n]);
-  // one graph at a time first (the plain form), then double the worker
-  // count while graphs/s improves by at least 2%
+  // one graph at a time (the plain form), then 2, 4, 8, ... in flight; the
+  // best rate wins (the curve is flat near its top, so stopping at the first
+  // drop would let one noisy trial decide)
   double best = 0.0;
   int32_t best_w = 1, k = 0;
   for (int32_t w = 1; w <= max_workers; w *= 2) {
@@ -170,11 +171,9 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
     if (tried) tried[k] = w;
     if (rates) rates[k] = rate;
     ++k;
-    if (rate > 1.02 * best) {
+    if (rate > best) {
       best = rate;
       best_w = w;
-    } else {
-      break;
     }
   }
   *workers = best_w;
