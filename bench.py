@@ -287,7 +287,7 @@ def batched_bench(n_graphs=256, workers=0, replay=0):
 
 
 def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interval_s=0.05,
-                 check=4, replay=0, final_partition=True, repartition_growth=2.0):
+                 check=4, replay=0, final_partition=False, repartition_growth=2.0):
     """The served stream: n_graphs DCSBMs (N log-uniform in [1000, 200000])
     arriving by a Poisson process at each offered rate, each delivered as 10
     increments of 10% of its shuffled edges, interval_s apart; stage 1
