@@ -18,7 +18,8 @@
  *   through the *_fx entry points, as its exact int128 (hi, lo) words.
  * Random stream: Philox4x64-10, key {seed, 0}, counter {item, proposal,
  *   stream, tag}; slots 0 neighbour, 1 accept-uniform, 2 uniform block,
- *   3 multinomial; an index in [0, W) is (x * W) >> 64.
+ *   3 multinomial; an index in [0, W) is (x * W) >> 64.  usap_set_uniforms
+ *   replaces it with a caller's pre-drawn stream.
  */
 #ifndef USAP_H
 #define USAP_H
@@ -211,6 +212,31 @@ typedef struct usap_sweep {
 } usap_sweep;
 int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, double mean,
                      double thr, int32_t check, usap_sweep* out);
+
+/* Deterministic harness mode: the engine draws from a caller's pre-drawn
+ * uniform stream instead of Philox.  Replaces the per-worker generators
+ * (_pt_generator, graph_parallel.hpp:67-77, :180) at their draw sites
+ * (merge :805-862, nodal :1022-1046), so that the reference, fed the same
+ * buffer, makes the same proposals.  u[n]: n/4 draws of four uniforms in
+ * [0, 1), slot order as the Philox words (neighbour, accept-uniform,
+ * uniform block, multinomial); a uniform u stands for the word floor(u 2^64),
+ * so an index in [0, W) is floor(u W) and the uniform branch tests
+ * u' <= B / (d + B) with u' = u rounded down to a multiple of 2^-53.
+ * Draws are consumed in call order from a cursor (reset to 0 here):
+ *   usap_merge_propose(K): block r, proposal j takes draw pos + r*K + j;
+ *     the round takes B*K draws;
+ *   usap_nodal_batch(begin, end): node n takes draw pos + (n - begin); the
+ *     batch takes end - begin draws;
+ *   usap_nodal_sweep: node n takes draw pos + n; the sweep takes one draw per
+ *     proposal evaluated (N if it runs to the end) and needs N available;
+ * so a sweep equals its batches called one by one, and usap_partition /
+ * usap_resume consume the stream in the reference's own order.  The stream
+ * arguments of those calls are ignored while the mode is on.  USAP_ESTATE
+ * (nothing run, cursor unchanged) when a call needs more draws than are left.
+ * u = NULL, n = 0 returns to Philox.  Copies u to the device. */
+int usap_set_uniforms(usap_engine* e, const double* u, size_t n);
+/* Draws consumed from the pre-drawn stream since usap_set_uniforms. */
+int usap_uniforms_used(usap_engine* e, int64_t* draws);
 
 /* Exact sum of the last batch: int128 words and inf flag (0 or -1). */
 int usap_nodal_sum_fx(usap_engine* e, int64_t* hi, uint64_t* lo, int32_t* inf);

@@ -90,6 +90,7 @@ def lib():
                                             C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                             C.POINTER(C.c_uint64)]
         L.orc_block_merge.argtypes = [C.c_void_p, C.c_uint64, _i32p, _i64p, _u64p]
+        L.orc_block_merge.restype = C.c_int
         L.orc_carry_out.argtypes = [C.c_void_p, C.c_int32]
         L.orc_nodal_propose_one.argtypes = [C.c_void_p, C.c_int32, C.c_uint64,
                                             C.POINTER(C.c_int32), C.POINTER(C.c_int64),
@@ -99,6 +100,11 @@ def lib():
                                       _i32p, _i64p, _u64p, _i32p, _i32p,
                                       C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.orc_nodal_batch.restype = C.c_int
+        L.orc_set_uniforms.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.orc_set_uniforms.restype = C.c_int
+        L.orc_uniforms_used.argtypes = [C.c_void_p]
+        L.orc_uniforms_used.restype = C.c_int64
         L.orc_nodal_eval.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64,
                                      C.POINTER(C.c_int32)]
         L.orc_merge_eval.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64]
@@ -269,7 +275,8 @@ class Oracle:
         s = np.zeros(B, np.int32)
         hi = np.zeros(B, np.int64)
         lo = np.zeros(B, np.uint64)
-        lib().orc_block_merge(self.st, stream, s, hi, lo)
+        if lib().orc_block_merge(self.st, stream, s, hi, lo):
+            raise RuntimeError("orc_block_merge: pre-drawn uniform stream exhausted")
         return s, [fx(h, l) for h, l in zip(hi, lo)]
 
     def carry_out(self, B_to_merge):
@@ -289,11 +296,27 @@ class Oracle:
         inf = np.zeros(max(n, 1), np.int32)
         acc = np.zeros(max(n, 1), np.int32)
         shi, slo, sinf, mv = C.c_int64(), C.c_uint64(), C.c_int32(), C.c_int32()
-        lib().orc_nodal_batch(self.st, beg, end, stream, s, hi, lo, inf, acc, C.byref(shi),
-                              C.byref(slo), C.byref(sinf), C.byref(mv))
+        if lib().orc_nodal_batch(self.st, beg, end, stream, s, hi, lo, inf, acc, C.byref(shi),
+                                 C.byref(slo), C.byref(sinf), C.byref(mv)):
+            raise RuntimeError("orc_nodal_batch: pre-drawn uniform stream exhausted")
         dS = [fx(h, l) for h, l in zip(hi[:n], lo[:n])]
         return dict(s=s[:n].copy(), dS=dS, inf=inf[:n].copy(), accept=acc[:n].copy(),
                     sum=fx(shi.value, slo.value), sum_inf=sinf.value, moves=mv.value)
+
+    def set_uniforms(self, u=None):
+        """Draw from a pre-drawn uniform stream (four uniforms in [0, 1) per
+        draw); None returns to Philox."""
+        if u is None:
+            rc = lib().orc_set_uniforms(self.st, None, 0)
+        else:
+            u = np.ascontiguousarray(u, np.float64).ravel()
+            self._uni = u
+            rc = lib().orc_set_uniforms(self.st, u.ctypes.data, u.size)
+        if rc:
+            raise ValueError("orc_set_uniforms: n must be a multiple of 4, values in [0, 1)")
+
+    def uniforms_used(self):
+        return lib().orc_uniforms_used(self.st)
 
     def nodal_eval(self, beg, end, stream):
         a = C.c_int32()

@@ -140,7 +140,19 @@ struct orc_state {
   /* per-phase record of the last orc_partition */
   orc_phase* plog;
   int32_t plog_n, plog_cap;
+  /* pre-drawn uniform stream (orc_set_uniforms): 4 words per draw, or NULL */
+  uint64_t* uni;
+  int64_t uni_n, uni_pos; /* draws held, draws consumed */
 };
+
+/* Draw d of the pre-drawn stream (its four words, Philox slot order). */
+static void pre_draw(const orc_state* st, uint64_t d, uint64_t x[4]) {
+  if (d >= (uint64_t)st->uni_n) abort(); /* callers reserve their draws first */
+  for (int j = 0; j < 4; ++j) x[j] = st->uni[4 * d + (uint64_t)j];
+}
+
+/* Draws left in the pre-drawn stream. */
+static int64_t uni_left(const orc_state* st) { return st->uni_n - st->uni_pos; }
 
 /* sgn * T(m, x, y) */
 static inline void acc_T(const orc_state* st, dsacc* a, int sgn, int64_t m, int64_t x,
@@ -331,8 +343,26 @@ void orc_destroy(orc_state* st) {
   free(st->col_idx);
   free(st->col_w);
   free(st->plog);
+  free(st->uni);
   free(st);
 }
+
+int orc_set_uniforms(orc_state* st, const double* u, int64_t n) {
+  if (n < 0 || n % 4 || (n && !u)) return -1;
+  for (int64_t i = 0; i < n; ++i)
+    if (!(u[i] >= 0.0 && u[i] < 1.0)) return -1;
+  free(st->uni);
+  st->uni = NULL;
+  st->uni_n = st->uni_pos = 0;
+  if (n) {
+    st->uni = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) st->uni[i] = (uint64_t)ldexp(u[i], 64); /* floor(u 2^64) */
+    st->uni_n = n / 4;
+  }
+  return 0;
+}
+
+int64_t orc_uniforms_used(const orc_state* st) { return st->uni_pos; }
 
 static inline int is_dense(const orc_state* st) { return st->B < st->p.block_size; }
 
@@ -518,7 +548,10 @@ static void merge_propose(orc_state* st, tscratch* ts, int32_t r, int32_t idx, u
   }
   int64_t k = k_out + k_in;
   uint64_t x[4];
-  draw4(st->p.seed, (uint64_t)r, (uint64_t)idx, stream, ORC_TAG_MERGE, x);
+  if (st->uni) /* pre-drawn: stream = the round's first draw */
+    pre_draw(st, stream + (uint64_t)r * (uint64_t)st->p.K + (uint64_t)idx, x);
+  else
+    draw4(st->p.seed, (uint64_t)r, (uint64_t)idx, stream, ORC_TAG_MERGE, x);
   int32_t s;
   if (k == 0) {
     s = (int32_t)idx_of(x[2], B);
@@ -570,6 +603,13 @@ static void merge_propose(orc_state* st, tscratch* ts, int32_t r, int32_t idx, u
 
 void orc_merge_propose_one(orc_state* st, int32_t r, int32_t idx, uint64_t stream, int32_t* s,
                            int64_t* dS_hi, uint64_t* dS_lo) {
+  if (st->uni) {
+    if ((int64_t)st->B * st->p.K > uni_left(st)) {
+      *s = -1;
+      return;
+    }
+    stream = (uint64_t)st->uni_pos;
+  }
   tscratch* ts = scratch_alloc(1, st->B);
   i128 d;
   merge_propose(st, ts, r, idx, stream, s, &d);
@@ -603,8 +643,13 @@ static void merge_item(void* vp, int tid, int64_t i) {
   st->best_dS[r] = bestd;
 }
 
-void orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s, int64_t* dS_hi,
-                     uint64_t* dS_lo) {
+int orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s, int64_t* dS_hi,
+                    uint64_t* dS_lo) {
+  if (st->uni) {
+    if ((int64_t)st->B * st->p.K > uni_left(st)) return -1;
+    stream = (uint64_t)st->uni_pos;
+    st->uni_pos += (int64_t)st->B * st->p.K;
+  }
   merge_ctx c;
   c.st = st;
   c.ts = scratch_alloc(st->p.threads, st->B);
@@ -616,12 +661,17 @@ void orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s, int64_t* d
     if (best_s) best_s[r] = st->best_s[r];
     if (dS_hi) split128(st->best_dS[r], &dS_hi[r], &dS_lo[r]);
   }
+  return 0;
 }
 
 void orc_merge_eval(orc_state* st, int32_t r0, int32_t r1, uint64_t stream) {
   if (r0 < 0) r0 = 0;
   if (r1 > st->B) r1 = st->B;
   if (r1 <= r0) return;
+  if (st->uni) {
+    if ((int64_t)st->B * st->p.K > uni_left(st)) return;
+    stream = (uint64_t)st->uni_pos;
+  }
   merge_ctx c;
   c.st = st;
   c.ts = scratch_alloc(st->p.threads, st->B);
@@ -707,7 +757,10 @@ static void nodal_propose(orc_state* st, tscratch* ts, int32_t n, uint64_t strea
   for (int64_t e = i0; e < i1; ++e) k_in += win(g, e);
   int64_t k = k_out + k_in;
   uint64_t x[4];
-  draw4(st->p.seed, (uint64_t)n, 0, stream, ORC_TAG_NODAL, x);
+  if (st->uni) /* pre-drawn: node n takes draw stream + n */
+    pre_draw(st, stream + (uint64_t)n, x);
+  else
+    draw4(st->p.seed, (uint64_t)n, 0, stream, ORC_TAG_NODAL, x);
   int32_t s;
   if (k == 0) {
     s = (int32_t)idx_of(x[2], B);
@@ -800,6 +853,15 @@ static void nodal_propose(orc_state* st, tscratch* ts, int32_t n, uint64_t strea
 
 void orc_nodal_propose_one(orc_state* st, int32_t n, uint64_t stream, int32_t* s, int64_t* dS_hi,
                            uint64_t* dS_lo, int32_t* inf, int32_t* accept) {
+  if (st->uni) {
+    if ((int64_t)n >= uni_left(st)) {
+      *s = -1;
+      *inf = 2;
+      *accept = 0;
+      return;
+    }
+    stream = (uint64_t)st->uni_pos;
+  }
   tscratch* ts = scratch_alloc(1, st->B);
   dsacc d;
   int acc;
@@ -841,10 +903,15 @@ static void nodal_run(orc_state* st, int32_t beg, int32_t end, uint64_t stream, 
 
 /* nodal_move + update, graph_parallel.hpp:406-458 (without the convergence
  * test, which is in orc_partition) */
-void orc_nodal_batch(orc_state* st, int32_t beg, int32_t end, uint64_t stream, int32_t* s_out,
-                     int64_t* dS_hi, uint64_t* dS_lo, int32_t* inf_out, int32_t* acc_out,
-                     int64_t* sum_hi, uint64_t* sum_lo, int32_t* sum_inf, int32_t* n_moves) {
+int orc_nodal_batch(orc_state* st, int32_t beg, int32_t end, uint64_t stream, int32_t* s_out,
+                    int64_t* dS_hi, uint64_t* dS_lo, int32_t* inf_out, int32_t* acc_out,
+                    int64_t* sum_hi, uint64_t* sum_lo, int32_t* sum_inf, int32_t* n_moves) {
   int32_t n = end - beg;
+  if (st->uni) {
+    if ((int64_t)n > uni_left(st)) return -1;
+    stream = (uint64_t)(st->uni_pos - beg);
+    st->uni_pos += n;
+  }
   int32_t* s = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   dsacc* d = (dsacc*)malloc(sizeof(dsacc) * (size_t)(n > 0 ? n : 1));
   int* acc = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -873,10 +940,15 @@ void orc_nodal_batch(orc_state* st, int32_t beg, int32_t end, uint64_t stream, i
   free(s);
   free(d);
   free(acc);
+  return 0;
 }
 
 void orc_nodal_eval(orc_state* st, int32_t beg, int32_t end, uint64_t stream, int32_t* n_accept) {
   int32_t n = end - beg;
+  if (st->uni) {
+    if ((int64_t)n > uni_left(st)) return;
+    stream = (uint64_t)(st->uni_pos - beg);
+  }
   int32_t* s = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
   dsacc* d = (dsacc*)malloc(sizeof(dsacc) * (size_t)(n > 0 ? n : 1));
   int* acc = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -1087,11 +1159,14 @@ static double state_entropy(void* ctx) {
 
 static void state_batch(void* ctx, int32_t beg, int32_t end, uint64_t stream, dsacc* sum,
                         int32_t* moves) {
-  int64_t hi;
-  uint64_t lo;
-  int32_t sinf;
-  orc_nodal_batch((orc_state*)ctx, beg, end, stream, NULL, NULL, NULL, NULL, NULL, &hi, &lo,
-                  &sinf, moves);
+  int64_t hi = 0;
+  uint64_t lo = 0;
+  int32_t sinf = 0;
+  *moves = 0;
+  /* a pre-drawn stream that runs out leaves the state unchanged; the
+   * partition checks the cursor after the phase */
+  (void)orc_nodal_batch((orc_state*)ctx, beg, end, stream, NULL, NULL, NULL, NULL, NULL, &hi,
+                        &lo, &sinf, moves);
   sum->inf = sinf;
   sum->fx = ((i128)hi << 64) | (i128)lo;
 }
@@ -1189,7 +1264,10 @@ int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats) {
   for (int64_t phase = 0;; ++phase) {
     /* prepare_merge + block_merge + perform_merge + prepare_move */
     nmerge += (double)st->B * p->K;
-    orc_block_merge(st, merge_stream++, NULL, NULL, NULL);
+    if (orc_block_merge(st, merge_stream++, NULL, NULL, NULL) < 0) {
+      err = -4; /* pre-drawn stream exhausted */
+      break;
+    }
     if (orc_carry_out(st, (int32_t)Btm) < 0) {
       err = -5;
       break;
@@ -1200,7 +1278,12 @@ int32_t orc_partition(orc_state* st, int32_t* labels_out, double* stats) {
     const int isf = isfinite(slots[0].S) && isfinite(slots[1].S) && isfinite(slots[2].S);
     nodal_ops ops = {st, state_entropy, state_batch};
     nodal_result nr;
+    const int64_t pos0 = st->uni_pos;
     nodal_phase(p, N, batch, isf, &ops, &nodal_stream, itr_dS, &nr);
+    if (st->uni && st->uni_pos - pos0 != nr.proposals) {
+      err = -4; /* pre-drawn stream exhausted */
+      break;
+    }
     nsweep += nr.sweeps;
     nbatch += (double)nr.batches;
     nprop += (double)nr.proposals;

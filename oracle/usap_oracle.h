@@ -110,9 +110,10 @@ void orc_entropy(const orc_state* st, int64_t* data_hi, uint64_t* data_lo, doubl
 /* One merge proposal of block r, proposal index idx. */
 void orc_merge_propose_one(orc_state* st, int32_t r, int32_t idx, uint64_t stream,
                            int32_t* s, int64_t* dS_hi, uint64_t* dS_lo);
-/* All blocks, K proposals each; best s and dS per block. */
-void orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s,
-                     int64_t* dS_hi, uint64_t* dS_lo);
+/* All blocks, K proposals each; best s and dS per block.  Returns 0, or -1
+ * (nothing done) when a pre-drawn stream has fewer than B*K draws left. */
+int orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s,
+                    int64_t* dS_hi, uint64_t* dS_lo);
 /* The K proposals of blocks [r0, r1) only (best per block kept in the
  * state, no output); times the CPU baseline on a bounded sample. */
 void orc_merge_eval(orc_state* st, int32_t r0, int32_t r1, uint64_t stream);
@@ -127,11 +128,13 @@ void orc_nodal_propose_one(orc_state* st, int32_t n, uint64_t stream, int32_t* s
                            int32_t* accept);
 /* Nodes [beg,end) against the frozen state, then apply the accepted moves and
  * rebuild M.  Per-node results go to the optional arrays (length end-beg).
- * Returns the sum of accepted dS (fixed point + inf flag) and the move count. */
-void orc_nodal_batch(orc_state* st, int32_t beg, int32_t end, uint64_t stream,
-                     int32_t* s_out, int64_t* dS_hi, uint64_t* dS_lo, int32_t* inf_out,
-                     int32_t* acc_out, int64_t* sum_hi, uint64_t* sum_lo,
-                     int32_t* sum_inf, int32_t* n_moves);
+ * Returns the sum of accepted dS (fixed point + inf flag) and the move count.
+ * Returns 0, or -1 (nothing done) when a pre-drawn stream has fewer than
+ * end-beg draws left. */
+int orc_nodal_batch(orc_state* st, int32_t beg, int32_t end, uint64_t stream,
+                    int32_t* s_out, int64_t* dS_hi, uint64_t* dS_lo, int32_t* inf_out,
+                    int32_t* acc_out, int64_t* sum_hi, uint64_t* sum_lo,
+                    int32_t* sum_inf, int32_t* n_moves);
 
 /* Same as orc_nodal_batch but only evaluates proposals (no apply); used to
  * time the CPU baseline on a bounded sample. */
@@ -187,6 +190,24 @@ int orc_nodal_phase_scripted(const orc_params* p, int32_t N, int32_t batch, int3
                              const uint64_t* sum_lo, const int32_t* sum_inf,
                              const int32_t* moves, int64_t n_batch, double* out,
                              double* itr_dS);
+
+/* Pre-drawn uniform stream in place of Philox (u: n/4 draws of four
+ * uniforms in [0,1), slot order as above; a uniform u stands for the word
+ * floor(u 2^64)).  Draws are taken from a cursor, reset to 0 here:
+ *   orc_block_merge: block r, proposal j takes draw pos + r*K + j; the round
+ *     takes B*K draws;
+ *   orc_nodal_batch(beg, end): node n takes draw pos + (n - beg); the batch
+ *     takes end - beg draws (so the batches of a sweep, in order, give node n
+ *     draw pos_sweep + n);
+ *   orc_merge_propose_one / orc_merge_eval read the next round's draws and
+ *   orc_nodal_propose_one (node n: pos + n) / orc_nodal_eval the next
+ *   batch's, without taking them.
+ * Stream arguments are ignored meanwhile; orc_partition returns -4 if the
+ * stream runs out.  u = NULL, n = 0 returns to Philox.  Returns 0, or -1 for
+ * a bad buffer (n not a multiple of 4, a value outside [0,1)). */
+int orc_set_uniforms(orc_state* st, const double* u, int64_t n);
+/* Draws consumed since orc_set_uniforms. */
+int64_t orc_uniforms_used(const orc_state* st);
 
 /* Textbook double-precision data_S = -sum m*log(m/(d_out*d_in)) (no fixed
  * point) for cross-checks. */

@@ -228,6 +228,8 @@ SIGNATURES = [
     ("usap_entropy", C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                C.POINTER(C.c_uint64)]),
     ("usap_entropy_noalloc", C.c_int, [_vp, _vp, _vp]),
+    ("usap_set_uniforms", C.c_int, [_vp, _vp, C.c_size_t]),
+    ("usap_uniforms_used", C.c_int, [_vp, _vp]),
     ("usap_merge_propose", C.c_int, [_vp, C.c_int32, C.c_uint64, _vp, _vp]),
     ("usap_merge_result_fx", C.c_int, [_vp, _i32p, _i64p, _u64p]),
     ("usap_merge_apply", C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int32)]),

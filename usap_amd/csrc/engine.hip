@@ -101,7 +101,7 @@ struct Status {
 // graph_parallel.hpp:433-458, run by block 0 of each batch's k_apply).
 struct Sweep {
   unsigned long long cum_lo, cum_hi;  // exact cumulative dS of the sweep (int128)
-  unsigned long long stream0;         // random stream of the sweep's first batch
+  unsigned long long stream0;         // random stream of the sweep's first batch (pre-drawn: node 0's draw)
   double S;                           // running S: S += dS_sum after every batch
   double mean;                        // moving average of earlier sweeps' dS
   double thr;                         // delta_entropy_threshold1 or 2
@@ -355,6 +355,17 @@ __device__ __forceinline__ void m_add(const MView& m, int32_t a, int32_t b, int3
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Draw d of a pre-drawn uniform stream: its four words, slot order as the
+// Philox outputs they replace.
+__device__ __forceinline__ void predrawn(const uint64_t* uni, uint64_t d, uint64_t x[4]) {
+  const ulonglong2* q = reinterpret_cast<const ulonglong2*>(uni + 4 * d);
+  const ulonglong2 a = q[0], b = q[1];
+  x[0] = a.x;
+  x[1] = a.y;
+  x[2] = b.x;
+  x[3] = b.y;
+}
 
 // Orders this wavefront's earlier table writes before its later reads.
 __device__ __forceinline__ void wave_fence() {
@@ -1043,7 +1054,8 @@ struct MergeArgs {
   const int64_t* lam;
   int32_t B;
   int32_t K;
-  uint64_t seed, stream;
+  uint64_t seed, stream;  // pre-drawn mode: stream = the round's first draw
+  const uint64_t* uni;    // pre-drawn stream (4 words per draw), else null
   int32_t* best_s;
   long* best_hi;
   unsigned long* best_lo;
@@ -1149,7 +1161,10 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   const int64_t k = dor + dir;
   uint64_t x[4];
-  usap::philox4x64((uint64_t)r, (uint64_t)idx, a.stream, usap::kTagMerge, a.seed, 0, x);
+  if (a.uni)
+    predrawn(a.uni, a.stream + (uint64_t)r * (uint64_t)a.K + (uint64_t)idx, x);
+  else
+    usap::philox4x64((uint64_t)r, (uint64_t)idx, a.stream, usap::kTagMerge, a.seed, 0, x);
   int32_t s;
   if (k == 0) {
     s = (int32_t)usap::idx_of(x[2], B);
@@ -1345,6 +1360,7 @@ struct NodalArgs {
   int32_t exact;
   int32_t bidx;  // batch index in the sweep: random stream sweep->stream0 + bidx
   uint64_t seed;
+  const uint64_t* uni;  // pre-drawn stream: node n takes draw sweep->stream0 + n; else null
   const int64_t* big_off;
   int32_t* big_tab;
   int32_t* prop_s;
@@ -1573,7 +1589,10 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
   const int32_t B = a.B;
   const int64_t kc = kco + kci;
   uint64_t x[4];
-  usap::philox4x64((uint64_t)n, 0, stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0, x);
+  if (a.uni)
+    predrawn(a.uni, stream0 + (uint64_t)n, x);
+  else
+    usap::philox4x64((uint64_t)n, 0, stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0, x);
   const Row rM = m_row(a.M, r);
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
   int64_t k_out = kco, k_in = kci;
@@ -2428,6 +2447,10 @@ struct usap_engine {
   int32_t* prop_inf = nullptr;
   int32_t* prop_acc = nullptr;
   int32_t last_beg = 0, last_end = 0;
+  // pre-drawn uniform stream (usap_set_uniforms): 4 words per draw, or null
+  uint64_t* uni = nullptr;
+  int64_t uni_n = 0;    // draws held
+  int64_t uni_pos = 0;  // draws consumed
   Status* status = nullptr;   // the running batch
   BlockPart* part = nullptr;  // per-workgroup shares of the running batch
   Sweep* sweep = nullptr;     // device sweep state
@@ -2918,6 +2941,7 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->aq_mark);
   dfree(e, e->aq_acc);
   dfree(e, e->build_flags);
+  dfree(e, e->uni);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   dfree(e, e->tstamp);
   pin_put(e->h_sweep, sizeof(Sweep));
@@ -3709,6 +3733,15 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   a.K = K;
   a.seed = e->p.seed;
   a.stream = stream;
+  a.uni = e->uni;
+  if (e->uni) {  // the round takes draws uni_pos + r*K + idx
+    const int64_t need = (int64_t)B * K;
+    if (need > e->uni_n - e->uni_pos)
+      return set_err(e, USAP_ESTATE, "pre-drawn uniform stream exhausted: a merge round needs " +
+                                         std::to_string(need) + " draws, " +
+                                         std::to_string(e->uni_n - e->uni_pos) + " left");
+    a.stream = (uint64_t)e->uni_pos;
+  }
   a.best_s = e->best_s;
   a.best_hi = e->best_hi;
   a.best_lo = e->best_lo;
@@ -3716,6 +3749,7 @@ int usap_merge_propose(usap_engine* e, int32_t K, uint64_t stream, int32_t* best
   ev_rec(e, 0);
   k_merge<<<B, nw * kWave, 0, e->stream>>>(a);
   HIPCHK(hipGetLastError());
+  if (e->uni) e->uni_pos += (int64_t)B * K;
   ev_rec(e, 1);
   if (e->prof) {  // the reference algorithm's bytes: 2 (4 (B+1) + 8 nnz(M)) + 24 B
     k_nnz_bytes<<<(unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, blocks_for(e->M.P, 256))),
@@ -3915,6 +3949,7 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
   a.prop_acc = e->prop_acc;
   a.prop_r = e->prop_r;
   a.prop_u = e->prof ? e->prop_u : nullptr;
+  a.uni = e->uni;
   a.part = e->part;
   a.status = e->status;
   a.sweep = e->sweep;
@@ -4120,6 +4155,13 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
                      int32_t* n_moves) {
   if (int rc = check_engine(e)) return rc;
   if (begin < 0 || end > e->N || begin > end) return USAP_EINVAL;
+  if (e->uni) {  // node n takes draw uni_pos + (n - begin)
+    if (end - begin > e->uni_n - e->uni_pos)
+      return set_err(e, USAP_ESTATE, "pre-drawn uniform stream exhausted: a nodal batch needs " +
+                                         std::to_string(end - begin) + " draws, " +
+                                         std::to_string(e->uni_n - e->uni_pos) + " left");
+    stream = (uint64_t)(e->uni_pos - begin);
+  }
   e->last_beg = begin;
   e->last_end = end;
   init_sweep(e, stream, 0.0, 0.0, 0.0, 0);
@@ -4139,6 +4181,7 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
       e->kp[USAP_K_APPLY].bytes += (double)hs.apply_bytes;
     }
     if (hs.stop == 3) return set_err(e, USAP_ESTATE, "inconsistent M during a nodal batch");
+    if (e->uni) e->uni_pos += end - begin;
     if (hs.flags & (kFlagLoad | kFlagFull))
       if (int rc = rebuild_M(e)) return rc;
   }
@@ -4151,6 +4194,35 @@ int usap_nodal_batch(usap_engine* e, int32_t begin, int32_t end, uint64_t stream
 int usap_nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, double S, double mean,
                      double thr, int32_t check, usap_sweep* out) {
   return usap::nodal_sweep(e, batch, stream0, &S, mean, thr, check, out);
+}
+
+int usap_set_uniforms(usap_engine* e, const double* u, size_t n) {
+  if (!e) return USAP_EINVAL;
+  if (n % 4 || (n && !u))
+    return set_err(e, USAP_EINVAL, "usap_set_uniforms: n must be a multiple of 4 (four uniforms per draw)");
+  std::vector<uint64_t> w(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (!(u[i] >= 0.0 && u[i] < 1.0))
+      return set_err(e, USAP_EINVAL, "usap_set_uniforms: u[" + std::to_string(i) + "] is not in [0, 1)");
+    w[i] = (uint64_t)std::ldexp(u[i], 64);  // floor(u * 2^64): exact below 2^64
+  }
+  dfree(e, e->uni);
+  e->uni_n = 0;
+  e->uni_pos = 0;
+  e->epoch++;  // a captured sweep holds the old pointer
+  if (n) {
+    if (int rc = dalloc(e, &e->uni, n)) return rc;
+    HIPCHK(hipMemcpyAsync(e->uni, w.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->uni_n = (int64_t)(n / 4);
+  }
+  return USAP_OK;
+}
+
+int usap_uniforms_used(usap_engine* e, int64_t* draws) {
+  if (!e || !draws) return USAP_EINVAL;
+  *draws = e->uni_pos;
+  return USAP_OK;
 }
 
 }  // extern "C"
@@ -4170,6 +4242,13 @@ int nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, const double* S
     S = &S0;
   }
   const int32_t nb = (int32_t)(((int64_t)e->N + batch - 1) / batch);
+  if (e->uni) {  // node n takes draw uni_pos + n; the sweep needs all N available
+    if (e->N > e->uni_n - e->uni_pos)
+      return set_err(e, USAP_ESTATE, "pre-drawn uniform stream exhausted: a nodal sweep needs " +
+                                         std::to_string(e->N) + " draws, " +
+                                         std::to_string(e->uni_n - e->uni_pos) + " left");
+    stream0 = (uint64_t)e->uni_pos;
+  }
   init_sweep(e, stream0, S ? *S : 0.0, mean, thr, check);
   HIPCHK(hipMemcpyAsync(e->sweep, e->h_sweep, sizeof(Sweep), hipMemcpyHostToDevice, e->stream));
   if (!S) {
@@ -4186,6 +4265,7 @@ int nodal_sweep(usap_engine* e, int32_t batch, uint64_t stream0, const double* S
   e->last_end = std::min<int64_t>((int64_t)e->last_beg + batch, e->N);
   out->batches = hs.done;
   out->proposals = std::min<int64_t>((int64_t)hs.done * batch, e->N);
+  if (e->uni) e->uni_pos += out->proposals;
   out->converged = hs.stop == 1;
   out->moves = (int64_t)hs.moves;
   out->S = hs.S;

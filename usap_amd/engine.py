@@ -157,6 +157,21 @@ class Engine:
         d["dS_fx"] = fx(d["dS_hi"], d["dS_lo"])
         return d
 
+    def set_uniforms(self, u=None):
+        """Draw from a pre-drawn uniform stream (usap_set_uniforms): u holds
+        four uniforms in [0, 1) per draw; None returns to Philox."""
+        if u is None:
+            self._ck(lib().usap_set_uniforms(self.h, None, 0))
+            return
+        u = np.ascontiguousarray(u, np.float64).ravel()
+        self._ck(lib().usap_set_uniforms(self.h, ptr(u), u.size))
+
+    def uniforms_used(self):
+        """Draws consumed from the pre-drawn stream (usap_uniforms_used)."""
+        d = C.c_int64()
+        self._ck(lib().usap_uniforms_used(self.h, C.byref(d)))
+        return d.value
+
     def snapshot(self, slot, S=0.0):
         self._ck(lib().usap_snapshot(self.h, slot, S))
 
