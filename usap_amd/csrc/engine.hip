@@ -76,6 +76,7 @@ constexpr int kTab = 512;                // per-wavefront LDS table slots
 constexpr int kTabMaxDeg = kTab * 3 / 4; // larger nodes use a global-memory table
 constexpr int kMergeWaves = 4;           // wavefronts per workgroup in k_merge
 constexpr int kNBK = 512;                // bucket sums per block row
+constexpr int kLookupRatio = 8;          // a hashed slab this many times a bucket is looked up per key
 constexpr int kScr = 3 * kTab;           // per-wavefront LDS scratch (ints) for one bucket
 constexpr int kDegLds = 1024;            // k_degrees privatises its counters for B <= this
 constexpr int kSmallB = 128;             // k_insert_small: B*B counts in LDS
@@ -492,6 +493,55 @@ __device__ __forceinline__ void gather_row(const MView& m, Row R, int32_t lo, in
   bytes += 12 * (long)R.c;
 }
 
+// scr[k - lo] = M[a][k] + M[k][a] for lo <= k < hi from hashed row R of a, by
+// looking each key up: four keys per lane at a time, every pending walk of
+// the wavefront advancing one slot per step with all its loads in flight.  A
+// walk ends at its key or at an empty slot, and after R.c slots at the latest
+// (a full slab).  For a slab much larger than the bucket this reads a few
+// slots per key instead of the whole slab.
+__device__ __forceinline__ void lookup_bucket(const MView& m, Row R, int32_t lo, int32_t hi,
+                                              int32_t* scr, long& bytes) {
+  const int lane = lane_id();
+  const uint32_t mask = (uint32_t)R.c - 1;
+  for (int32_t base = lo; base < hi; base += 4 * kWave) {
+    uint32_t h[4];
+    int32_t wv[4];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t key = base + q * kWave + lane;
+      h[q] = hash32(key) & mask;
+      wv[q] = 0;
+      if (key < hi) pend |= 1u << q;
+    }
+    for (int32_t step = 0; step < R.c && __ballot(pend != 0); ++step) {
+      Slot x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((pend >> q) & 1) x[q] = m.kv[R.o + h[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        bytes += 16 * (long)__popcll(__ballot((pend >> q) & 1));
+        if (!((pend >> q) & 1)) continue;
+        const int32_t key = base + q * kWave + lane;
+        if (x[q].k == key) {
+          wv[q] = x[q].o + x[q].i;
+          pend &= ~(1u << q);
+        } else if (x[q].k < 0) {
+          pend &= ~(1u << q);
+        } else {
+          h[q] = (h[q] + 1) & mask;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t key = base + q * kWave + lane;
+      if (key < hi) scr[key - lo] = wv[q];
+    }
+  }
+}
+
 // Index of the t-th unit in the LDS weights scr[0, width) (t < their sum):
 // each lane sums a contiguous run of entries, one wavefront scan finds the
 // run, and the lane owning it walks it.
@@ -529,7 +579,9 @@ __device__ __forceinline__ int32_t scan_scratch(const int32_t* scr, int32_t widt
 // the bucket is at most a wavefront wide, else they are gathered into the
 // wavefront's LDS scratch `scr` (scr_cap ints) -- a hashed row by scanning its
 // slab, which costs a few loads per lane where looking up each absent key
-// would walk a probe chain -- and searched with one scan.
+// would walk a probe chain, or, when the slab holds kLookupRatio times the
+// bucket's keys or more, by looking each key up (a whole-slab scan of a
+// 16384-slot row took one wavefront ~120 us) -- and searched with one scan.
 __device__ __forceinline__ int32_t select_combined(const MView& M, int32_t u, int64_t t,
                                                    int32_t excl, int32_t* scr, int32_t scr_cap,
                                                    long& bytes) {
@@ -607,6 +659,8 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, int32_t u, in
         }
       }
       bytes += 8 * (long)width;
+    } else if (RM.c >= kLookupRatio * width) {
+      lookup_bucket(M, RM, lo, hi, scr, bytes);
     } else {
       for (int32_t i = lane; i < width; i += kWave) scr[i] = 0;
       wave_fence();
