@@ -2075,10 +2075,20 @@ __device__ __forceinline__ Table apply_table(const ApplyArgs& a, int32_t n, int3
         if (bn[q] == -2) {
           tab_add(T.tk, T.two, mask, bo[q], w);
         } else {  // the neighbour moved in this batch too: bo -> bn
-          m_add<true>(a.M, r, bo[q], -w, 0, flags);
-          m_add<true>(a.M, bo[q], r, 0, -w, flags);
-          m_add<true>(a.M, s, bn[q], w, 0, flags);
-          m_add<true>(a.M, bn[q], s, 0, w, flags);
+          // four independent updates: their rows, then their first keys,
+          // are read together before the adds
+          const int32_t ra[4] = {r, bo[q], s, bn[q]}, kb[4] = {bo[q], r, bn[q], s};
+          const int32_t dO[4] = {-w, 0, w, 0}, dI[4] = {0, -w, 0, w};
+          Row R4[4];
+          int64_t sl[4];
+          int32_t k4[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) R4[i] = m_row(a.M, ra[i]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) k4[i] = m_first_key(a.M, R4[i], kb[i], sl[i]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            m_add_at<true>(a.M, ra[i], R4[i], kb[i], dO[i], dI[i], sl[i], k4[i], flags);
         }
       } else {
         wi_sum += w;
