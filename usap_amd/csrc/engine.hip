@@ -2032,19 +2032,19 @@ __device__ void sweep_tail(Sweep* sw, u128 sum, int32_t ninf, int32_t moves,
 // column r for row and column s.  Edges to neighbours that stay put are first
 // summed per neighbour block b in the node's table (wo = edges n -> b, wi =
 // edges b -> n), so each changed pair of counts gets one 64-bit atomic per
-// node: block row r at key b (-wo, -wi), row s at b (+wo, +wi), row b at r
-// (-wi, -wo) and at s (+wi, +wo); the four first keys are read together
-// before the adds.  Old labels of nodes inside the batch come from prop_r, so
-// writing labels cannot race with their reads.  Returns the algorithmic bytes.
+// node (apply_entry); edges to neighbours that move in the same batch are
+// applied here, one by one.  Also moves the node's degrees and writes its
+// label.  Old labels of nodes inside the batch come from prop_r, so writing
+// labels cannot race with their reads.  Returns the table, compacted to its
+// nent entries (nent = 0 when none is left).
 template <bool kBig>
-__device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t r, int32_t s,
-                                          int64_t o0, int64_t kco, int64_t i0, int64_t kci,
-                                          int32_t* tab) {
+__device__ __forceinline__ Table apply_table(const ApplyArgs& a, int32_t n, int32_t r, int32_t s,
+                                             int64_t o0, int64_t kco, int64_t i0, int64_t kci,
+                                             int32_t* tab, int32_t& nent) {
   const int lane = lane_id();
   const GraphView& g = a.g;
   int32_t* flags = &a.status->flags;
   const int64_t kc = kco + kci;
-  const Row rM = m_row(a.M, r), sM = m_row(a.M, s);
   const Table T = tab_open<kBig>(tab, a.big_off, a.big_tab, n, kc);
   const uint32_t mask = (uint32_t)T.cap - 1;
   long wo_sum = 0, wi_sum = 0;
@@ -2075,10 +2075,20 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
         if (bn[q] == -2) {
           tab_add(T.tk, T.two, mask, bo[q], w);
         } else {  // the neighbour moved in this batch too: bo -> bn
-          m_add<true>(a.M, r, bo[q], -w, 0, flags);
-          m_add<true>(a.M, bo[q], r, 0, -w, flags);
-          m_add<true>(a.M, s, bn[q], w, 0, flags);
-          m_add<true>(a.M, bn[q], s, 0, w, flags);
+          // four independent updates: their rows, then their first keys,
+          // are read together before the adds
+          const int32_t ra[4] = {r, bo[q], s, bn[q]}, kb[4] = {bo[q], r, bn[q], s};
+          const int32_t dO[4] = {-w, 0, w, 0}, dI[4] = {0, -w, 0, w};
+          Row R4[4];
+          int64_t sl[4];
+          int32_t k4[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) R4[i] = m_row(a.M, ra[i]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) k4[i] = m_first_key(a.M, R4[i], kb[i], sl[i]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            m_add_at<true>(a.M, ra[i], R4[i], kb[i], dO[i], dI[i], sl[i], k4[i], flags);
         }
       } else {
         wi_sum += w;
@@ -2089,22 +2099,7 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
   }
   wave_fence();
   long x0 = 0, x1 = 0, x2 = 0, x3 = 0;
-  const int32_t nent = tab_compact(T, -2, -2, x0, x1, x2, x3);
-  for (int32_t j = lane; j < nent; j += kWave) {
-    const int32_t b = T.tk[j], wo = T.two[j], wi = T.twi[j];
-    if (wo == 0 && wi == 0) continue;
-    const Row bM = m_row(a.M, b);
-    int64_t sl[4];
-    int32_t k[4];
-    k[0] = m_first_key(a.M, rM, b, sl[0]);  // (M[r][b], M[b][r]) -= (wo, wi)
-    k[1] = m_first_key(a.M, sM, b, sl[1]);  // (M[s][b], M[b][s]) += (wo, wi)
-    k[2] = m_first_key(a.M, bM, r, sl[2]);  // (M[b][r], M[r][b]) -= (wi, wo)
-    k[3] = m_first_key(a.M, bM, s, sl[3]);  // (M[b][s], M[s][b]) += (wi, wo)
-    m_add_at<true>(a.M, r, rM, b, -wo, -wi, sl[0], k[0], flags);
-    m_add_at<true>(a.M, s, sM, b, wo, wi, sl[1], k[1], flags);
-    m_add_at<true>(a.M, b, bM, r, -wi, -wo, sl[2], k[2], flags);
-    m_add_at<true>(a.M, b, bM, s, wi, wo, sl[3], k[3], flags);
-  }
+  nent = tab_compact(T, -2, -2, x0, x1, x2, x3);
   wo_sum = wave_sum(wo_sum);
   wi_sum = wave_sum(wi_sum);
   if (lane == 0) {
@@ -2120,7 +2115,26 @@ __device__ __forceinline__ long apply_one(const ApplyArgs& a, int32_t n, int32_t
     atomicAdd(&a.bsize[s], 1);
     a.labels[n] = s;
   }
-  return apply_tally(kc);
+  return T;
+}
+
+// One entry (b, wo, wi) of a moving node's table, node moving r -> s: block
+// row r at key b (-wo, -wi), row s at b (+wo, +wi), row b at r (-wi, -wo)
+// and at s (+wi, +wo); the four first keys are read together before the adds.
+__device__ __forceinline__ void apply_entry(const ApplyArgs& a, int32_t r, Row rM, int32_t s,
+                                            Row sM, int32_t b, int32_t wo, int32_t wi) {
+  int32_t* flags = &a.status->flags;
+  const Row bM = m_row(a.M, b);
+  int64_t sl[4];
+  int32_t k[4];
+  k[0] = m_first_key(a.M, rM, b, sl[0]);  // (M[r][b], M[b][r]) -= (wo, wi)
+  k[1] = m_first_key(a.M, sM, b, sl[1]);  // (M[s][b], M[b][s]) += (wo, wi)
+  k[2] = m_first_key(a.M, bM, r, sl[2]);  // (M[b][r], M[r][b]) -= (wi, wo)
+  k[3] = m_first_key(a.M, bM, s, sl[3]);  // (M[b][s], M[s][b]) += (wi, wo)
+  m_add_at<true>(a.M, r, rM, b, -wo, -wi, sl[0], k[0], flags);
+  m_add_at<true>(a.M, s, sM, b, wo, wi, sl[1], k[1], flags);
+  m_add_at<true>(a.M, b, bM, r, -wi, -wo, sl[2], k[2], flags);
+  m_add_at<true>(a.M, b, bM, s, wi, wo, sl[3], k[3], flags);
 }
 
 // The batch's sums over all workgroups' shares (nparts of them), by one
@@ -2173,8 +2187,16 @@ __device__ __forceinline__ void batch_tail(const BlockPart* part, uint32_t npart
 
 // Block 0 runs the batch's update step from k_nodal's shares while blocks
 // 1.. apply the moves of items (block - 1) * kNodalWaves + wave.
+// Block 0 runs the batch's update step; every other block applies the moves
+// of its four batch nodes: each moving node's wavefront builds its table,
+// then the entries of all the block's tables are spread over all its
+// threads (a block holds 0-4 moving nodes, most often 0 or 1, so a node's
+// entries no longer wait for its own 64 lanes only).
 __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   __shared__ int32_t s_tab[kNodalWaves][3 * kTab];
+  __shared__ int32_t* s_tk[kNodalWaves];  // each moving node's compacted table (LDS or global)
+  __shared__ int32_t s_stride[kNodalWaves];
+  __shared__ int32_t s_n[kNodalWaves], s_r[kNodalWaves], s_s[kNodalWaves];
   if (blockIdx.x == 0) {
     if (a.tstamp && threadIdx.x == 0) a.tstamp[0] = wall_clock64();
     if (a.sweep->stop_n == 0) batch_tail(a.part, a.nparts, a.sweep);
@@ -2198,11 +2220,38 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   const int64_t kco = a.g.out_ptr[n + 1] - o0, kci = a.g.in_ptr[n + 1] - i0;
   if (stop) return;
   if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
+  int32_t nent = 0;
+  Table T{s_tab[w], s_tab[w] + kTab, s_tab[w] + 2 * kTab, kTab};
   if (in && acc) {
     if (kco + kci > kTabMaxDeg)
-      apply_one<true>(a, n, pr, ps, o0, kco, i0, kci, s_tab[w]);
+      T = apply_table<true>(a, n, pr, ps, o0, kco, i0, kci, s_tab[w], nent);
     else
-      apply_one<false>(a, n, pr, ps, o0, kco, i0, kci, s_tab[w]);
+      T = apply_table<false>(a, n, pr, ps, o0, kco, i0, kci, s_tab[w], nent);
+  }
+  if (lane_id() == 0) {
+    s_n[w] = nent;
+    s_r[w] = pr;
+    s_s[w] = ps;
+    s_tk[w] = T.tk;
+    s_stride[w] = (int32_t)(T.two - T.tk);
+  }
+  __syncthreads();
+  int32_t off[kNodalWaves + 1];
+  off[0] = 0;
+#pragma unroll
+  for (int i = 0; i < kNodalWaves; ++i) off[i + 1] = off[i] + s_n[i];
+  const int32_t total = off[kNodalWaves];
+  for (int32_t e = threadIdx.x; e < total; e += blockDim.x) {
+    int wv = 0;
+#pragma unroll
+    for (int i = 1; i < kNodalWaves; ++i) wv += e >= off[i];
+    const int32_t j = e - off[wv];
+    const int32_t* tk = s_tk[wv];
+    const int32_t st = s_stride[wv];
+    const int32_t b = tk[j], wo = tk[st + j], wi = tk[2 * st + j];
+    if (wo == 0 && wi == 0) continue;
+    const int32_t r = s_r[wv], s = s_s[wv];
+    apply_entry(a, r, m_row(a.M, r), s, m_row(a.M, s), b, wo, wi);
   }
   if (a.tstamp) {
     __syncthreads();
