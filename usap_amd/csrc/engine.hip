@@ -1546,12 +1546,15 @@ __device__ __forceinline__ void list_entry(const GraphView& g, int64_t o0, int64
 __device__ __forceinline__ i128 nodal_delta(const NodalArgs& a, const Table& T, int32_t r,
                                             int32_t s, Row rM, int64_t dor, int64_t dir,
                                             int64_t kc, int64_t k_out, int64_t k_in, long wself,
-                                            bool& inf, long& bytes) {
+                                            bool& inf, long& bytes, int32_t s_pre, Row sM_pre,
+                                            int64_t dos_pre, int64_t dis_pre) {
   const int lane = lane_id();
   const int64_t* lam = a.lam;
-  // s's row and degrees; the table is compacted while they load
-  const Row sM = m_row(a.M, s);
-  const int64_t dos = a.d_out[s], dis = a.d_in[s];
+  // s's row and degrees (loaded ahead when s is the uniform draw); the table
+  // is compacted while they load
+  const bool pre = s == s_pre;
+  const Row sM = pre ? sM_pre : m_row(a.M, s);
+  const int64_t dos = pre ? dos_pre : a.d_out[s], dis = pre ? dis_pre : a.d_in[s];
   long wor = 0, wir = 0, wos = 0, wis = 0;
   const int32_t nent = tab_compact(T, r, s, wor, wir, wos, wis);
   // corner: (r, r), (r, s) with (s, r) beside it, (s, s)
@@ -1649,6 +1652,11 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
     usap::philox4x64((uint64_t)n, 0, stream0 + (uint64_t)a.bidx, usap::kTagNodal, a.seed, 0, x);
   const Row rM = m_row(a.M, r);
   const int64_t dor = a.d_out[r], dir = a.d_in[r];
+  // the uniform draw's block, the proposal in most cases: its row and
+  // degrees load with r's
+  const int32_t s_pre = (int32_t)usap::idx_of(x[2], B);
+  const Row sM_pre = m_row(a.M, s_pre);
+  const int64_t dos_pre = a.d_out[s_pre], dis_pre = a.d_in[s_pre];
   int64_t k_out = kco, k_in = kci;
   if (g.out_w) {
     long so = 0, si = 0;
@@ -1718,7 +1726,8 @@ __device__ __forceinline__ long nodal_one(const NodalArgs& a, int64_t item, int3
   int32_t acc = 0;
   if (s != r) {
     wave_fence();  // the table is complete
-    dS = nodal_delta(a, T, r, s, rM, dor, dir, kc, k_out, k_in, wself, inf, bytes);
+    dS = nodal_delta(a, T, r, s, rM, dor, dir, kc, k_out, k_in, wself, inf, bytes, s_pre, sM_pre,
+                     dos_pre, dis_pre);
     acc = (inf || dS < 0) ? 1 : 0;
   }
   if (lane == 0) {
