@@ -1115,6 +1115,12 @@ struct MergeArgs {
   unsigned long* best_lo;
 };
 
+// L(m1) + L(m2) - L(m1+m2) when both counts are non-zero, else 0.
+__device__ __forceinline__ i128 overlap_pair(const int64_t* lam, int32_t m1, int32_t m2) {
+  return (m1 > 0 && m2 > 0) ? Lterm(lam, m1) + Lterm(lam, m2) - Lterm(lam, (int64_t)m1 + m2)
+                            : (i128)0;
+}
+
 // per-lane part of sum_{i not in {r,s}} [L(m1) + L(m2) - L(m1+m2)] over the
 // keys where rows r and s of M are both non-zero, and the same over the
 // columns (the only terms that survive the factorised merge dS): one pass
@@ -1126,6 +1132,15 @@ __device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r
   const Row R1 = m_row(m, a1), R2 = m_row(m, a2);
   const bool direct = R1.c == m.B;
   i128 acc = 0;
+  if (direct) {  // then so is the longer row: key j sits at slot j of both
+    for (int32_t j = lane; j < R1.c; j += kWave) {
+      const int2 c1 = *reinterpret_cast<const int2*>(&m.kv[R1.o + j].o);
+      const int2 c2 = *reinterpret_cast<const int2*>(&m.kv[R2.o + j].o);
+      if (j == r || j == s) continue;
+      acc += overlap_pair(lam, c1.x, c2.x) + overlap_pair(lam, c1.y, c2.y);
+    }
+    return acc;
+  }
   for (int32_t j = lane; j < R1.c; j += kWave) {
     const Slot q = m.kv[R1.o + j];
     const int32_t i = direct ? j : q.k;
@@ -1261,7 +1276,7 @@ __device__ void merge_one(const MergeArgs& a, int32_t r, int32_t idx, int32_t* s
   dS_out = old_c - new_c + acc + deg;
 }
 
-__global__ __launch_bounds__(kWave* kMergeWaves) void k_merge(MergeArgs a) {
+__global__ __launch_bounds__(kWave* kMergeWaves, 5) void k_merge(MergeArgs a) {
   __shared__ long s_hi[kMergeWaves];
   __shared__ unsigned long s_lo[kMergeWaves];
   __shared__ int32_t s_idx[kMergeWaves];
