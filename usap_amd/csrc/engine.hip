@@ -1141,13 +1141,27 @@ __device__ i128 merge_overlap_lane(const MView& m, const int64_t* lam, int32_t r
     }
     return acc;
   }
-  for (int32_t j = lane; j < R1.c; j += kWave) {
-    const Slot q = m.kv[R1.o + j];
-    const int32_t i = direct ? j : q.k;
-    if (i < 0 || (q.o <= 0 && q.i <= 0) || i == r || i == s) continue;
-    const Cnt c2 = m_value(m, R2, i, m_probe(m, R2, i));
-    if (q.o > 0 && c2.o > 0) acc += Lterm(lam, q.o) + Lterm(lam, c2.o) - Lterm(lam, (int64_t)q.o + c2.o);
-    if (q.i > 0 && c2.i > 0) acc += Lterm(lam, q.i) + Lterm(lam, c2.i) - Lterm(lam, (int64_t)q.i + c2.i);
+  // hashed shorter row: kOvl of its slots per lane, all their probes of
+  // the other row issued before any is resolved
+  constexpr int kOvl = 4;
+  for (int32_t base = 0; base < R1.c; base += kOvl * kWave) {
+    Slot q[kOvl];
+#pragma unroll
+    for (int u = 0; u < kOvl; ++u) {
+      const int32_t j = base + u * kWave + lane;
+      q[u] = j < R1.c ? m.kv[R1.o + j] : Slot{-1, 0, 0, 0};
+      if (q[u].k < 0 || (q[u].o <= 0 && q[u].i <= 0) || q[u].k == r || q[u].k == s) q[u].k = -1;
+    }
+    Probe p[kOvl];
+#pragma unroll
+    for (int u = 0; u < kOvl; ++u)
+      if (q[u].k >= 0) p[u] = m_probe(m, R2, q[u].k);
+#pragma unroll
+    for (int u = 0; u < kOvl; ++u) {
+      if (q[u].k < 0) continue;
+      const Cnt c2 = m_value(m, R2, q[u].k, p[u]);
+      acc += overlap_pair(lam, q[u].o, c2.o) + overlap_pair(lam, q[u].i, c2.i);
+    }
   }
   return acc;
 }
