@@ -191,3 +191,25 @@ def test_new_buffer_recaptures_sweep(usap):
     ref.nodal_sweep(40, 0, ref.entropy()[1], 0.0, 0.0, 0)
     np.testing.assert_array_equal(eng.labels(), ref.labels())
     assert eng.entropy()[0] == ref.entropy()[0]
+
+
+def test_partition_c2(usap, orc):
+    """C2 (N = 20000) from the reference's default start on one shared
+    pre-drawn stream (about 1.8 M draws): labels, B, S and counts."""
+    g = usap.config_graph("c2")
+    params = dict(seed=7)
+    U = np.random.default_rng(16).random(4 * 3_000_000)
+    o = orc.Oracle(g, threads=16, **params)
+    o.set_labels(*o.scc())
+    o.set_uniforms(U)
+    B, olab, ost = o.partition()
+    assert ost["error"] == 0, ost
+    eng = usap.Engine(g, **params)
+    eng.set_uniforms(U)
+    lab, st = eng.partition()
+    assert st["B"] == B and st["S"] == ost["S"]
+    np.testing.assert_array_equal(lab, olab)
+    assert (st["sweeps"], st["batches"], st["nodal_proposals"]) == \
+        (ost["sweeps"], ost["batches"], ost["nodal_proposals"])
+    assert eng.uniforms_used() == o.uniforms_used() == \
+        st["merge_proposals"] + st["nodal_proposals"]
