@@ -543,7 +543,10 @@ def main():
         log(f"[bench] SCC init: B={sst['B']} S={sst['S']:.1f} nmi={out['scc_init']['nmi']}")
         out["planted"] = {"B": int(g.truth.max()) + 1,
                           "S": planted_S(usap_amd, g, params)}
-        out["batched"] = batched_child(args.graphs, args.workers)
+        # 8 in flight, the autotuner's usual pick: the autotuner itself (its
+        # trial runs create and destroy many worker streams and memory pools)
+        # has ended in host crashes and is only run with --config c4
+        out["batched"] = batched_child(args.graphs, args.workers or 8)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, params, plog, args.cpu_seconds)
         out["vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
