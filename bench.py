@@ -97,23 +97,32 @@ def cpu_baseline(g, params, phase_log, seconds, threads=16):
         if eng.B <= final_B:
             break
     eng.close()
-    per = seconds / max(1, 2 * len(states))
+    per = seconds / max(1, 3 * len(states))
 
     def timed(fn, n0, nmax):
+        # grow the sample until it takes about per/2, then time that size once
+        # more and keep the faster run: a host hiccup during one of the two
+        # (another process on the host's cores) would otherwise price a whole
+        # range of B at a fraction of the CPU's real rate
         n = n0
         while True:
             t = time.perf_counter()
             fn(n)
             el = time.perf_counter() - t
             if el > per / 2 or n >= nmax:
-                return n, el
+                t = time.perf_counter()
+                fn(n)
+                return n, min(el, time.perf_counter() - t)
             n = min(nmax, int(n * max(2.0, (per / 2) / max(el, 1e-3))))
 
+    # samples start at 8 items per thread: the oracle runs fewer than 64
+    # items on one thread, and a few items per thread leave threads idle
+    n0 = max(64, 8 * threads)
     rates = []
     for B, lab in states:
         o.set_labels(lab, B)
-        nm, em = timed(lambda m: o.merge_eval(0, m, 0), 16, B)
-        nn, en = timed(lambda n: o.nodal_eval(0, n, 0), 256, g.N)
+        nm, em = timed(lambda m: o.merge_eval(0, m, 0), min(n0, B), B)
+        nn, en = timed(lambda n: o.nodal_eval(0, n, 0), min(max(n0, 256), g.N), g.N)
         rates.append((B, K * nm / em, nn / en))
         log(f"[bench] cpu sample B={B}: {K * nm} merge proposals in {em:.2f}s, "
             f"{nn} nodal proposals in {en:.2f}s ({threads} threads)")

@@ -114,6 +114,8 @@ static inline void acc_inf(dsacc* a, int sgn) {
     a->inf = 2;
 }
 
+struct tscratch;
+
 struct orc_state {
   orc_graph g;
   orc_params p;
@@ -143,6 +145,10 @@ struct orc_state {
   /* pre-drawn uniform stream (orc_set_uniforms): 4 words per draw, or NULL */
   uint64_t* uni;
   int64_t uni_n, uni_pos; /* draws held, draws consumed */
+  /* per-worker scratch kept between calls (scratch_get) */
+  struct tscratch* ts;
+  int ts_threads;
+  int32_t ts_B;
 };
 
 /* Draw d of the pre-drawn stream (its four words, Philox slot order). */
@@ -248,7 +254,7 @@ static void par_for(int threads, int64_t n, item_fn fn, void* ctx) {
 }
 
 /* per-worker dense scratch: NewM (graph_parallel.hpp:126-140) + prob */
-typedef struct {
+typedef struct tscratch {
   int64_t *Rr, *Rs, *Cr, *Cs, *prob;
 } tscratch;
 
@@ -273,6 +279,22 @@ static void scratch_free(tscratch* ts, int threads) {
     free(ts[t].prob);
   }
   free(ts);
+}
+
+/* The state's scratch, reused by every call and allocated again only for
+ * more threads or a larger B.  Buffers allocated per call would make a
+ * call's time depend on the allocator (the page faults of freshly mapped
+ * memory when the heap has been trimmed) rather than on the algorithm.
+ * Nothing reads scratch it has not written first (every proposal clears the
+ * rows it uses), so reuse leaves every result as it was. */
+static tscratch* scratch_get(orc_state* st, int threads) {
+  if (st->ts && st->ts_threads >= threads && st->ts_B >= st->B) return st->ts;
+  if (st->ts) scratch_free(st->ts, st->ts_threads);
+  const int t = threads > st->p.threads ? threads : st->p.threads;
+  st->ts = scratch_alloc(t, st->B);
+  st->ts_threads = t;
+  st->ts_B = st->B;
+  return st->ts;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -344,6 +366,7 @@ void orc_destroy(orc_state* st) {
   free(st->col_w);
   free(st->plog);
   free(st->uni);
+  if (st->ts) scratch_free(st->ts, st->ts_threads);
   free(st);
 }
 
@@ -610,11 +633,10 @@ void orc_merge_propose_one(orc_state* st, int32_t r, int32_t idx, uint64_t strea
     }
     stream = (uint64_t)st->uni_pos;
   }
-  tscratch* ts = scratch_alloc(1, st->B);
+  tscratch* ts = scratch_get(st, 1);
   i128 d;
   merge_propose(st, ts, r, idx, stream, s, &d);
   split128(d, dS_hi, dS_lo);
-  scratch_free(ts, 1);
 }
 
 typedef struct {
@@ -652,11 +674,10 @@ int orc_block_merge(orc_state* st, uint64_t stream, int32_t* best_s, int64_t* dS
   }
   merge_ctx c;
   c.st = st;
-  c.ts = scratch_alloc(st->p.threads, st->B);
+  c.ts = scratch_get(st, st->p.threads);
   c.stream = stream;
   c.first = 0;
   par_for(st->p.threads, st->B, merge_item, &c);
-  scratch_free(c.ts, st->p.threads);
   for (int32_t r = 0; r < st->B; ++r) {
     if (best_s) best_s[r] = st->best_s[r];
     if (dS_hi) split128(st->best_dS[r], &dS_hi[r], &dS_lo[r]);
@@ -674,11 +695,10 @@ void orc_merge_eval(orc_state* st, int32_t r0, int32_t r1, uint64_t stream) {
   }
   merge_ctx c;
   c.st = st;
-  c.ts = scratch_alloc(st->p.threads, st->B);
+  c.ts = scratch_get(st, st->p.threads);
   c.stream = stream;
   c.first = r0;
   par_for(st->p.threads, r1 - r0, merge_item, &c);
-  scratch_free(c.ts, st->p.threads);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -862,14 +882,13 @@ void orc_nodal_propose_one(orc_state* st, int32_t n, uint64_t stream, int32_t* s
     }
     stream = (uint64_t)st->uni_pos;
   }
-  tscratch* ts = scratch_alloc(1, st->B);
+  tscratch* ts = scratch_get(st, 1);
   dsacc d;
   int acc;
   nodal_propose(st, ts, n, stream, s, &d, &acc);
   split128(d.fx, dS_hi, dS_lo);
   *inf = d.inf;
   *accept = acc;
-  scratch_free(ts, 1);
 }
 
 typedef struct {
@@ -891,14 +910,13 @@ static void nodal_run(orc_state* st, int32_t beg, int32_t end, uint64_t stream, 
                       dsacc* d, int* acc) {
   nodal_ctx c;
   c.st = st;
-  c.ts = scratch_alloc(st->p.threads, st->B);
+  c.ts = scratch_get(st, st->p.threads);
   c.stream = stream;
   c.beg = beg;
   c.s = s;
   c.d = d;
   c.acc = acc;
   par_for(st->p.threads, end - beg, nodal_item, &c);
-  scratch_free(c.ts, st->p.threads);
 }
 
 /* nodal_move + update, graph_parallel.hpp:406-458 (without the convergence
