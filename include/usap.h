@@ -77,9 +77,11 @@ typedef struct usap_params {
   int32_t slab_slack;              /* hashed block row capacity = pow2 >= slack*(out+in edges)+16; a row grows past 1/slack load (2; 1..16) */
   int32_t graphs;                  /* 1: replay each nodal sweep as a captured HIP graph (0 =
                                       default).  The capture is thread-local
-                                      (hipStreamCaptureModeThreadLocal) and the engine's own
-                                      thread makes no other HIP call while it lasts, so other
-                                      engines may run on other threads meanwhile */
+                                      (hipStreamCaptureModeThreadLocal).  Use it with one
+                                      engine at a time per process: usap_partition_batch and
+                                      usap_serve refuse it (USAP_EINVAL) with more than one
+                                      worker, since a process replaying six engines' sweeps
+                                      on six host threads once aborted (cause not found) */
 } usap_params;
 
 /* Partition statistics (usap_partition).  Times are host wall-clock ms. */
@@ -305,8 +307,9 @@ typedef struct usap_batch_stats {
  * engine and HIP stream, largest graphs first.  Graph i writes its N_i labels
  * to labels + label_off[i] and its statistics to stats[i] (nullable).  Each
  * graph's result is the one it gets alone.  workers = 0: chosen by
- * usap_batch_autotune.  Returns USAP_ESTATE if any graph failed (see
- * stats[i].error and out->failed). */
+ * usap_batch_autotune.  USAP_EINVAL if a graph asks for captured-sweep
+ * replay (params[i].graphs) and workers != 1.  Returns USAP_ESTATE if any
+ * graph failed (see stats[i].error and out->failed). */
 int usap_partition_batch(int32_t n, const usap_graph* graphs, const usap_params* params,
                          int32_t* labels, const int64_t* label_off, usap_stats* stats,
                          int32_t workers, usap_batch_stats* out);
@@ -380,8 +383,9 @@ int64_t usap_footprint(int32_t N, int64_t E);
  * hbm_budget; up to `workers` ready stages of different graphs run at once,
  * oldest first.  Each graph's final labels go to labels + label_off[g];
  * stages[g*stages + k-1] records stage k.  Every result equals the one the
- * graph gets served alone.  USAP_ESTATE if a graph failed (its stage record
- * holds the error). */
+ * graph gets served alone.  USAP_EINVAL if a graph asks for captured-sweep
+ * replay (params.graphs) with more than one worker.  USAP_ESTATE if a graph
+ * failed (its stage record holds the error). */
 int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts* opt,
                int32_t* labels, const int64_t* label_off, usap_serve_stage* stages,
                usap_serve_stats* out);
@@ -389,7 +393,8 @@ int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts*
 /* Picks the number of graphs in flight: partitions a sample of `sample`
  * graphs spread over the size range (0 = 24) with 1, 2, 4, 8, ... workers (at
  * most max_workers, 0 = 16) and takes the count with the best graphs/s.  tried /
- * rates (nullable, >= 16 entries) receive each trial's workers and graphs/s. */
+ * rates (nullable, >= 16 entries) receive each trial's workers and graphs/s.
+ * USAP_EINVAL if a graph asks for captured-sweep replay and max_workers != 1. */
 int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* params,
                         int32_t max_workers, int32_t sample, int32_t* workers, int32_t* tried,
                         double* rates);

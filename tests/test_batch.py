@@ -102,7 +102,26 @@ def test_batch_autotune(usap):
     assert bs["failed"] == 0 and 1 <= bs["workers"] <= 16
 
 
+def test_batch_refuses_replay_in_flight(usap):
+    """Captured-sweep replay is refused with more than one graph in flight
+    (checked before any device call); one at a time keeps it."""
+    from usap_amd import _lib
+    from usap_amd._lib import UsapError
+    graphs, specs = usap.batch_graphs(n=3, seed=12, n_min=300, n_max=600)
+    params = [dict(seed=s["seed"], graphs=1) for s in specs]
+    for workers in (0, 2, 3):
+        with pytest.raises(UsapError) as ei:
+            usap.partition_batch(graphs, params, workers=workers)
+        assert ei.value.code == _lib.USAP_EINVAL and "workers=1" in str(ei.value)
+    with pytest.raises(UsapError):
+        usap.autotune(graphs, params, max_workers=4)
+
+
 @pytest.mark.gpu
+@pytest.mark.skip(reason="six engines capturing and replaying their sweeps on six host threads "
+                         "ended the test process in SIGABRT once (no GPU fault, no runtime "
+                         "message); no cause found, so the batched entry points now refuse "
+                         "graphs=1 with more than one worker and the case stays off the GPU")
 def test_batch_with_graph_replay(usap):
     """Several engines in flight, each replaying its sweeps from a captured
     HIP graph: same partitions as alone without graphs."""

@@ -33,6 +33,12 @@ def test_serve_argument_checks(usap):
                 _lib.UsapServeOpts(10, 0.0, 0, 0), _lib.UsapServeOpts(10, 0.0, 2, -5)):
         assert L.usap_serve(0, None, C.byref(bad), None, None, None, None) == _lib.USAP_EINVAL
     assert L.usap_serve(1, None, C.byref(opt), None, None, None, None) == _lib.USAP_EINVAL
+    # captured-sweep replay with several stages in flight is refused before
+    # any device call
+    gs = sv.served_graphs(n=2, seed=8, n_min=300, n_max=400)
+    with pytest.raises(_lib.UsapError) as ei:
+        sv.serve(gs, [0.0, 0.0], dict(seed=1, graphs=1), stages=2, workers=2)
+    assert ei.value.code == _lib.USAP_EINVAL and "workers=1" in str(ei.value)
 
 
 def _small():

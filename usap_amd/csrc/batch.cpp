@@ -108,6 +108,18 @@ bool valid(int32_t n, const usap_graph* graphs, const usap_params* params) {
   return true;
 }
 
+// Captured-sweep replay (usap_params.graphs) is refused with several graphs
+// in flight: a test process running six engines that capture and replay
+// their sweeps on six host threads once ended in SIGABRT (no GPU fault, no
+// message from the runtime), and no cause was found.  One graph at a time
+// keeps it.
+bool replay_in_flight(int32_t n, const usap_params* params, int32_t workers) {
+  if (workers == 1 || n < 2) return false;
+  for (int32_t i = 0; i < n; ++i)
+    if (params[i].graphs) return true;
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -118,6 +130,11 @@ int usap_partition_batch(int32_t n, const usap_graph* graphs, const usap_params*
   if (!valid(n, graphs, params) || (n > 0 && (!labels || !label_off))) return USAP_EINVAL;
   if (workers < 0) return USAP_EINVAL;
   usap::clear_detached_error();
+  if (replay_in_flight(n, params, workers)) {
+    usap::set_detached_error(
+        "usap_partition_batch: graphs=1 (captured-sweep replay) needs workers=1");
+    return USAP_EINVAL;
+  }
   if (workers == 0) {
     const int rc = usap_batch_autotune(n, graphs, params, 0, 0, &workers, nullptr, nullptr);
     if (rc != USAP_OK) return rc;
@@ -146,6 +163,11 @@ int usap_batch_autotune(int32_t n, const usap_graph* graphs, const usap_params* 
   if (max_workers == 0) max_workers = 16;
   if (sample == 0) sample = 24;
   usap::clear_detached_error();
+  if (replay_in_flight(n, params, max_workers)) {
+    usap::set_detached_error(
+        "usap_batch_autotune: graphs=1 (captured-sweep replay) needs max_workers=1");
+    return USAP_EINVAL;
+  }
   *workers = 1;
   if (n == 0) return USAP_OK;
   // the sample spans the size range: every k-th graph of the size order

@@ -248,6 +248,14 @@ int usap_serve(int32_t n, const usap_serve_graph* graphs, const usap_serve_opts*
         graphs[i].arrival_s < 0)
       return USAP_EINVAL;
   usap::clear_detached_error();
+  // captured-sweep replay only with one stage in flight, as in
+  // usap_partition_batch
+  if (opt->workers > 1 && n > 1)
+    for (int32_t i = 0; i < n; ++i)
+      if (graphs[i].params.graphs) {
+        usap::set_detached_error("usap_serve: graphs=1 (captured-sweep replay) needs workers=1");
+        return USAP_EINVAL;
+      }
   Server sv;
   sv.n = n;
   sv.graphs = graphs;
