@@ -627,6 +627,10 @@ __device__ __forceinline__ int32_t select_combined(const MView& M, int32_t u, in
   const int32_t lo = jb << sh;
   const int32_t hi = min(B, lo + (1 << sh));
   const int32_t width = hi - lo;
+  // a bucket of one key (every bucket when B <= kNBK): the bucket's sum is
+  // that key's weight, so the bucket is the pick (the scans below would
+  // return lo too, also for the excluded key's emptied bucket)
+  if (width == 1) return lo;
   if (width <= kWave && RM.c == B) {  // direct row: one load per key
     long w = 0;
     const int32_t i = lo + lane;
