@@ -2306,9 +2306,12 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
 // row whose live keys and parked increments fit 1/(2 slack) of its slab (and
 // at most kCompact slots) is compacted in place through LDS; any other row
 // moves to a slab >= 2*slack*live+16 (direct once that reaches B/8) at the
-// free tail of the pool, and the row's parked increments are replayed into it.  Bucket
+// free tail of the pool, and the row's parked increments are replayed into it.  A
+// new slab of at most kGrowLds slots is laid out in LDS (its slots claimed
+// meanwhile) and written out whole; the old one is left as zeros.  Bucket
 // sums are unchanged.  Block 0 also acts on the batch's flags.
 constexpr int kCompact = 4096;
+constexpr int kGrowLds = 2048;    // slabs up to this many slots are laid out in LDS
 constexpr int kGrowBlocks = 512;  // k_grow's grid
 constexpr int32_t kOvfCap = 1 << 22;  // parked increments
 
@@ -2375,10 +2378,39 @@ __device__ void replay_row(const GrowSide& g, int32_t a, int64_t o, int32_t c, i
   }
 }
 
+// The live entries s_k/s_o/s_i[0, live) laid out as a slab of nc slots in
+// LDS t_k/t_o/t_i (hashed, or by key when direct), for one coalesced
+// write-out: no per-entry CAS round trip to the new slab.
+__device__ void lds_slab(const int32_t* s_k, const int32_t* s_o, const int32_t* s_i,
+                         int32_t live, int32_t* t_k, int32_t* t_o, int32_t* t_i, int32_t nc,
+                         bool direct) {
+  for (int32_t j = threadIdx.x; j < nc; j += blockDim.x) {
+    t_k[j] = -1;
+    t_o[j] = 0;
+    t_i[j] = 0;
+  }
+  __syncthreads();
+  const uint32_t mask = (uint32_t)nc - 1;
+  for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
+    const int32_t k = s_k[i];
+    uint32_t h = (uint32_t)k;
+    if (!direct) {
+      h = hash32(k) & mask;
+      while (atomicCAS(&t_k[h], -1, k) != -1) h = (h + 1) & mask;
+    }
+    t_o[h] = s_o[i];
+    t_i[h] = s_i[i];
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
-  __shared__ int32_t s_k[kCompact];
+  __shared__ int32_t s_k[kCompact];  // the row's live entries
   __shared__ int32_t s_o[kCompact];
   __shared__ int32_t s_i[kCompact];
+  __shared__ int32_t t_k[kGrowLds];  // its new slab, when it fits
+  __shared__ int32_t t_o[kGrowLds];
+  __shared__ int32_t t_i[kGrowLds];
   __shared__ long long s_base;
   __shared__ int32_t s_cap, s_live;
   const int32_t B = ga.B;
@@ -2403,43 +2435,46 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     const int32_t a = g.gq[qi];
     const int64_t o = g.off[a];
     const int32_t c = g.cap[a];
+    const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
     const bool dold = c == B;
+    // a hashed slab of at most kCompact slots has its live entries gathered
+    // into LDS by the counting pass
+    const bool gather = !dold && c <= kCompact;
     if (threadIdx.x == 0) s_live = 0;
     __syncthreads();
     for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
       const Slot x = g.kv[o + j];
       const int32_t k = dold ? j : x.k;
-      if (k >= 0 && (x.o != 0 || x.i != 0)) atomicAdd(&s_live, 1);
-    }
-    __syncthreads();
-    const int32_t live = s_live;
-    const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
-    int64_t o2 = o;                   // the row's slab after this step
-    int32_t c2 = c;
-    if (!dold && 2 * g.slack * (live + pend) <= c && c <= kCompact) {
-      // in place: gather the live entries, clear, reinsert
-      __syncthreads();
-      if (threadIdx.x == 0) s_live = 0;
-      __syncthreads();
-      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-        const Slot x = g.kv[o + j];
-        if (x.k >= 0 && (x.o != 0 || x.i != 0)) {
-          const int32_t i = atomicAdd(&s_live, 1);
-          s_k[i] = x.k;
+      if (k >= 0 && (x.o != 0 || x.i != 0)) {
+        const int32_t i = atomicAdd(&s_live, 1);
+        if (gather) {
+          s_k[i] = k;
           s_o[i] = x.o;
           s_i[i] = x.i;
         }
       }
-      __syncthreads();
-      for (int32_t j = threadIdx.x; j < c; j += blockDim.x) g.kv[o + j] = Slot{-1, 0, 0, 0};
-      __threadfence_block();
-      __syncthreads();
-      const uint32_t mask = (uint32_t)c - 1;
-      for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
-        uint32_t h = hash32(s_k[i]) & mask;
-        while (atomicCAS(&g.kv[o + h].k, -1, s_k[i]) != -1) h = (h + 1) & mask;
-        g.kv[o + h].o = s_o[i];
-        g.kv[o + h].i = s_i[i];
+    }
+    __syncthreads();
+    const int32_t live = s_live;
+    int64_t o2 = o;  // the row's slab after this step
+    int32_t c2 = c;
+    if (gather && 2 * g.slack * (live + pend) <= c) {
+      // in place: the slab rewritten with the live entries only
+      if (c <= kGrowLds) {
+        lds_slab(s_k, s_o, s_i, live, t_k, t_o, t_i, c, false);
+        for (int32_t j = threadIdx.x; j < c; j += blockDim.x)
+          g.kv[o + j] = Slot{t_k[j], 0, t_o[j], t_i[j]};
+      } else {
+        for (int32_t j = threadIdx.x; j < c; j += blockDim.x) g.kv[o + j] = Slot{-1, 0, 0, 0};
+        __threadfence_block();
+        __syncthreads();
+        const uint32_t mask = (uint32_t)c - 1;
+        for (int32_t i = threadIdx.x; i < live; i += blockDim.x) {
+          uint32_t h = hash32(s_k[i]) & mask;
+          while (atomicCAS(&g.kv[o + h].k, -1, s_k[i]) != -1) h = (h + 1) & mask;
+          g.kv[o + h].o = s_o[i];
+          g.kv[o + h].i = s_i[i];
+        }
       }
       __threadfence_block();
       __syncthreads();
@@ -2449,61 +2484,72 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
         g.gst->compactions += 1;
       }
     } else {
+      int64_t nc = 32;
+      while (nc < 2 * g.slack * ((int64_t)live + pend) + 16) nc <<= 1;
+      if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
+      if (8 * nc >= B) nc = B;
+      // the slots are claimed while the new slab is laid out in LDS; a
+      // direct row never moves
+      unsigned long long base = 0;
+      if (threadIdx.x == 0 && !dold) base = atomicAdd(&g.gst->used, (unsigned long long)nc);
+      const bool inlds = gather && nc <= kGrowLds;
+      if (inlds) lds_slab(s_k, s_o, s_i, live, t_k, t_o, t_i, (int32_t)nc, nc == B);
       if (threadIdx.x == 0) {
-        int64_t nc = 32;
-        while (nc < 2 * g.slack * ((int64_t)live + pend) + 16) nc <<= 1;
-        if (nc < 2 * (int64_t)c) nc = 2 * (int64_t)c;
-        if (8 * nc >= B) nc = B;
-        const unsigned long long base = atomicAdd(&g.gst->used, (unsigned long long)nc);
-        if (dold || (int64_t)base + nc > g.P) {
-          atomicAdd(&g.gst->used, (unsigned long long)(-nc));  // give the slots back
-          s_cap = 0;
-          if (!dold) {
+        s_cap = 0;
+        if (!dold) {
+          if ((int64_t)base + nc > g.P) {
+            atomicAdd(&g.gst->used, (unsigned long long)(-nc));  // give the slots back
             g.gst->nospace = 1;
             g.gst->nospace_events += 1;
             atomicOr(&ga.sweep->flags, kFlagFull);
             atomicCAS(&ga.sweep->stop, 0, 2);
+          } else {
+            s_cap = (int32_t)nc;
+            s_base = (long long)base;
           }
-        } else {
-          s_cap = (int32_t)nc;
-          s_base = (long long)base;
         }
       }
       __syncthreads();
-      const int32_t nc = s_cap;
-      if (nc > 0) {
+      const int32_t ncap = s_cap;
+      if (ncap > 0) {
         const int64_t nb = s_base;
-        for (int32_t i = threadIdx.x; i < nc; i += blockDim.x) g.kv[nb + i] = Slot{-1, 0, 0, 0};
-        __threadfence_block();
-        __syncthreads();
-        const bool dnew = nc == B;
-        const uint32_t mask = (uint32_t)nc - 1;
-        for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
-          const Slot x = g.kv[o + j];
-          if (x.k < 0 || (x.o == 0 && x.i == 0)) continue;
+        if (inlds) {
+          for (int32_t j = threadIdx.x; j < ncap; j += blockDim.x)
+            g.kv[nb + j] = Slot{t_k[j], 0, t_o[j], t_i[j]};
           // the old slab stays in the pool as zeros until a rebuild
-          g.kv[o + j].o = 0;
-          g.kv[o + j].i = 0;
-          if (dnew) {
-            g.kv[nb + x.k].o = x.o;
-            g.kv[nb + x.k].i = x.i;
-            continue;
+          for (int32_t j = threadIdx.x; j < c; j += blockDim.x) g.kv[o + j] = Slot{-1, 0, 0, 0};
+        } else {
+          for (int32_t i = threadIdx.x; i < ncap; i += blockDim.x) g.kv[nb + i] = Slot{-1, 0, 0, 0};
+          __threadfence_block();
+          __syncthreads();
+          const bool dnew = ncap == B;
+          const uint32_t mask = (uint32_t)ncap - 1;
+          for (int32_t j = threadIdx.x; j < c; j += blockDim.x) {
+            const Slot x = g.kv[o + j];
+            if (x.k < 0 || (x.o == 0 && x.i == 0)) continue;
+            g.kv[o + j].o = 0;
+            g.kv[o + j].i = 0;
+            if (dnew) {
+              g.kv[nb + x.k].o = x.o;
+              g.kv[nb + x.k].i = x.i;
+              continue;
+            }
+            uint32_t h = hash32(x.k) & mask;
+            while (atomicCAS(&g.kv[nb + h].k, -1, x.k) != -1) h = (h + 1) & mask;
+            g.kv[nb + h].o = x.o;
+            g.kv[nb + h].i = x.i;
           }
-          uint32_t h = hash32(x.k) & mask;
-          while (atomicCAS(&g.kv[nb + h].k, -1, x.k) != -1) h = (h + 1) & mask;
-          g.kv[nb + h].o = x.o;
-          g.kv[nb + h].i = x.i;
         }
         __threadfence_block();
         __syncthreads();
         if (threadIdx.x == 0) {
           g.off[a] = nb;
-          g.cap[a] = nc;
+          g.cap[a] = ncap;
           g.nkeys[a] = live;
           g.gst->grows += 1;
         }
         o2 = nb;
-        c2 = nc;
+        c2 = ncap;
       }
     }
     if (pend > 0) replay_row(g, a, o2, c2, B, ga.sweep);
