@@ -2703,6 +2703,16 @@ int set_err(usap_engine* e, int code, const std::string& msg) {
       return set_err(e, USAP_EHIP, std::string(#expr) + ": " + hipGetErrorString(_st)); \
   } while (0)
 
+// The runtime's resource calls (stream-ordered allocations and frees, pools,
+// streams, events, pinned host buffers, graph instantiation) are made under
+// one process-wide lock.  Engines on several host threads (batched and served
+// graphs) make these calls concurrently; test processes running several
+// engines on host threads have ended in crashes inside native threads with
+// no GPU fault and no cause found in this library, so the calls that create,
+// grow or release runtime state are kept from overlapping.  They are short
+// and few per graph; kernel launches and copies are not locked.
+std::mutex g_hip_mu;
+
 // Device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the
 // engine's stream): creating and destroying engines never synchronises the
 // device, so many engines can come and go while others run (batched and
@@ -2723,8 +2733,10 @@ int dalloc(usap_engine* e, T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
   e->epoch++;
+  std::unique_lock<std::mutex> lk(g_hip_mu);
   hipError_t st = e->pool ? hipMallocFromPoolAsync((void**)p, n * sizeof(T), e->pool, e->stream)
                           : hipMallocAsync((void**)p, n * sizeof(T), e->stream);
+  lk.unlock();
   if (st != hipSuccess) {
     *p = nullptr;
     return set_err(e, USAP_ENOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(st));
@@ -2736,8 +2748,10 @@ int dalloc(usap_engine* e, T** p, size_t n) {
 int dalloc_raw(usap_engine* e, void** p, size_t bytes) {
   *p = nullptr;
   e->epoch++;
+  std::unique_lock<std::mutex> lk(g_hip_mu);
   hipError_t st = e->pool ? hipMallocFromPoolAsync(p, bytes ? bytes : 1, e->pool, e->stream)
                           : hipMallocAsync(p, bytes ? bytes : 1, e->stream);
+  lk.unlock();
   if (st != hipSuccess) {
     *p = nullptr;
     return set_err(e, USAP_ENOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(st));
@@ -2747,7 +2761,10 @@ int dalloc_raw(usap_engine* e, void** p, size_t bytes) {
 
 template <typename T>
 void dfree(usap_engine* e, T*& p) {
-  if (p) (void)hipFreeAsync((void*)p, e->stream);
+  if (p) {
+    std::lock_guard<std::mutex> lk(g_hip_mu);
+    (void)hipFreeAsync((void*)p, e->stream);
+  }
   p = nullptr;
 }
 
@@ -2767,6 +2784,7 @@ void* pin_get(size_t bytes) {
       }
   }
   void* q = nullptr;
+  std::lock_guard<std::mutex> lk(g_hip_mu);
   if (hipHostMalloc(&q, bytes) != hipSuccess) return nullptr;
   return q;
 }
@@ -3133,7 +3151,10 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->aq_acc);
   dfree(e, e->build_flags);
   dfree(e, e->uni);
-  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->gexec) {
+    std::lock_guard<std::mutex> lk(g_hip_mu);
+    (void)hipGraphExecDestroy(e->gexec);
+  }
   dfree(e, e->tstamp);
   pin_put(e->h_sweep, sizeof(Sweep));
   for (int i = 0; i < 2; ++i) pin_put(e->h_chunk[i], sizeof(Sweep));
@@ -3142,13 +3163,14 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->ent_part);
   pin_put(e->h_ent_part, sizeof(long) * 2 * kEntropyBlocks);
   for (int i = 0; i < kNumSnap; ++i) dfree(e, e->snap_labels[i]);
-  for (hipEvent_t& ev : e->ev)
-    if (ev) (void)hipEventDestroy(ev);
-  for (hipEvent_t& ev : e->chunk_ev)
-    if (ev) (void)hipEventDestroy(ev);
-  if (e->stream) {
-    (void)hipStreamSynchronize(e->stream);
-    if (e->own_stream) (void)hipStreamDestroy(e->stream);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  {
+    std::lock_guard<std::mutex> lk(g_hip_mu);
+    for (hipEvent_t& ev : e->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t& ev : e->chunk_ev)
+      if (ev) (void)hipEventDestroy(ev);
+    if (e->stream && e->own_stream) (void)hipStreamDestroy(e->stream);
   }
   delete e;
   return USAP_OK;
@@ -3534,8 +3556,11 @@ int alloc_state(usap_engine* e) {
   HIPCHK(hipMemsetAsync(e->sweep, 0, sizeof(Sweep), st));
   HIPCHK(hipMemsetAsync(e->M.gst, 0, sizeof(GrowState), st));
   std::memset(e->h_sweep, 0, sizeof(Sweep));
-  for (hipEvent_t& ev : e->ev) HIPCHK(hipEventCreate(&ev));
-  for (hipEvent_t& ev : e->chunk_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  {
+    std::lock_guard<std::mutex> lk(g_hip_mu);
+    for (hipEvent_t& ev : e->ev) HIPCHK(hipEventCreate(&ev));
+    for (hipEvent_t& ev : e->chunk_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
   return USAP_OK;
 }
 
@@ -3560,13 +3585,31 @@ struct WorkerCtx {
   hipMemPool_t pool = nullptr;
 };
 
+// Worker contexts are kept for the life of the process: a released one goes
+// back to a free list (its stream synchronised, its pool trimmed to what is
+// in use) and the next worker on that device takes it, so repeated batches
+// (the autotuner's trials, one batch after another) create no further
+// streams or pools and never destroy one while other workers run.
+std::mutex g_ctx_mu;
+std::vector<WorkerCtx*> g_ctx_free;
+
 int worker_ctx_create(int32_t device, WorkerCtx** out) {
   if (!out) return USAP_EINVAL;
   *out = nullptr;
   if (hipSetDevice(device) != hipSuccess) return USAP_EHIP;
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    for (size_t i = 0; i < g_ctx_free.size(); ++i)
+      if (g_ctx_free[i]->device == device) {
+        *out = g_ctx_free[i];
+        g_ctx_free.erase(g_ctx_free.begin() + (long)i);
+        return USAP_OK;
+      }
+  }
   WorkerCtx* w = new (std::nothrow) WorkerCtx();
   if (!w) return USAP_ENOMEM;
   w->device = device;
+  std::lock_guard<std::mutex> lk(g_hip_mu);
   if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
     delete w;
     return USAP_EHIP;
@@ -3590,9 +3633,12 @@ void worker_ctx_destroy(WorkerCtx* w) {
   if (!w) return;
   (void)hipSetDevice(w->device);
   (void)hipStreamSynchronize(w->stream);
-  (void)hipMemPoolDestroy(w->pool);
-  (void)hipStreamDestroy(w->stream);
-  delete w;
+  {
+    std::lock_guard<std::mutex> lk(g_hip_mu);
+    (void)hipMemPoolTrimTo(w->pool, 0);  // the blocks its engines freed go back to the device
+  }
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  g_ctx_free.push_back(w);
 }
 
 int create_in(WorkerCtx* ctx, const usap_graph* g, const usap_params* p, usap_engine** out) {
@@ -3629,7 +3675,10 @@ int create_in(WorkerCtx* ctx, const usap_graph* g, const usap_params* p, usap_en
     e->own_stream = false;
   } else {
     prepare_pool(e->p.device);
-    hst = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    {
+      std::lock_guard<std::mutex> lk(g_hip_mu);
+      hst = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    }
     if (hst != hipSuccess) {
       e->stream = nullptr;
       usap_destroy(e);
@@ -4225,6 +4274,7 @@ int sweep_graph(usap_engine* e, int32_t batch, int32_t nb) {
   if (!e->gexec || e->g_B != e->B || e->g_batch != batch || e->g_nb != nb ||
       e->g_epoch != e->epoch) {
     if (e->gexec) {
+      std::lock_guard<std::mutex> lk(g_hip_mu);
       (void)hipGraphExecDestroy(e->gexec);
       e->gexec = nullptr;
     }
@@ -4237,6 +4287,7 @@ int sweep_graph(usap_engine* e, int32_t batch, int32_t nb) {
     }
     hipGraph_t graph = nullptr;
     const hipError_t ce = hipStreamEndCapture(st, &graph);
+    std::unique_lock<std::mutex> lk(g_hip_mu);
     if (rc != USAP_OK) {
       if (ce == hipSuccess && graph) (void)hipGraphDestroy(graph);
       return rc;
@@ -4244,6 +4295,7 @@ int sweep_graph(usap_engine* e, int32_t batch, int32_t nb) {
     HIPCHK(ce);
     const hipError_t ie = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
+    lk.unlock();
     if (ie != hipSuccess) {
       e->gexec = nullptr;
       return set_err(e, USAP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));

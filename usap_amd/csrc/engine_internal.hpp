@@ -29,8 +29,11 @@ void engine_set_error(usap_engine* e, const char* msg);
 void set_detached_error(const std::string& msg);
 // A host worker's device context: one HIP stream and one stream-ordered
 // memory pool that the engines the worker creates one after another share
-// (batched and served graphs), so that no stream or pool is created or
-// destroyed per graph and no freed block moves between workers' streams.
+// (batched graphs), so that no stream or pool is created or destroyed per
+// graph and no freed block moves between workers' streams.  Contexts live as
+// long as the process: worker_ctx_destroy synchronises the stream, trims the
+// pool and returns the context to a free list that worker_ctx_create takes
+// from first.
 struct WorkerCtx;
 int worker_ctx_create(int32_t device, WorkerCtx** out);
 void worker_ctx_destroy(WorkerCtx* w);
