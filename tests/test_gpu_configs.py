@@ -280,3 +280,27 @@ def test_c3_full_size_state_invariants(usap):
         if B in check_at:
             _flush(f"c3 check B={B}")
             _check_state(usap, eng, g, lam)
+
+
+def test_row_build_large_B_weighted(usap):
+    """The per-row M build (B >= 16384) on a weighted graph with self-loops,
+    hubs and isolated nodes, with blocks of one node to thousands: small rows
+    laid out in LDS, hashed rows above the LDS slab and a direct row inserted
+    in place.  M, the degrees and the MDL equal the host recompute after the
+    build and after a nodal sweep that moves nodes between such rows."""
+    g = random_graph(40000, 400000, seed=21, weighted=True, self_loops=True, hubs=3,
+                     isolated=50)
+    lam = usap.lambda_table(int(g.out_w.sum()))
+    B = 20000
+    lab = np.empty(g.N, np.int32)
+    lab[:3000] = 0                                # a direct row (the hubs are here)
+    lab[3000:3800] = 1 + np.arange(800) % 20      # hashed rows of 2048 slots
+    lab[3800:] = np.arange(g.N - 3800) % B        # one or two nodes: rows in LDS
+    eng = usap.Engine(g, seed=5, nodal_batch=4096)
+    eng.set_labels(lab, B)
+    _flush(f"row build B={eng.B}")
+    _check_state(usap, eng, g, lam)
+    sw = eng.nodal_sweep(4096, 0, 0.0, 0.0, 0.0, 0)
+    assert sw["moves"] > 0
+    _check_state(usap, eng, g, lam)
+    eng.close()

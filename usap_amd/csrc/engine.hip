@@ -23,8 +23,8 @@
 // The host rebuilds M from the labels only when the pool runs out.
 //
 // Kernels (one wavefront of 64 lanes per proposal item):
-//   k_degrees / k_caps / k_fill / k_insert_edges : M build from labels
-//        (graph_parallel.hpp:703-759)
+//   k_degrees / k_caps / k_fill / k_insert_* / k_build_rows : M build from
+//        labels (graph_parallel.hpp:703-759)
 //   k_merge   : K merge proposals per block + best per block
 //        (graph_parallel.hpp:349-368, :761-988), one workgroup per block,
 //        one wavefront per proposal
@@ -797,6 +797,14 @@ __global__ void k_fill(Slot* kv, int64_t n) {
 
 // Edge u -> v between blocks a -> b: o of row a at key b and i of row b at
 // key a (one update when a == b).
+// A row that k_build_rows lays out in LDS: hashed, at most kRowTab slots.
+constexpr int kRowTab = 1024;
+__device__ __forceinline__ bool lds_row(int32_t cap, int32_t B) { return cap < B && cap <= kRowTab; }
+
+// kSkipLds: only the rows k_build_rows does not build (direct, or hashed
+// above kRowTab slots), which may hold many nodes: edge by edge keeps them
+// load-balanced.
+template <bool kSkipLds>
 __global__ void k_insert_edges(int64_t E, const int32_t* __restrict__ src,
                                const int32_t* __restrict__ dst, const int32_t* __restrict__ w,
                                const int32_t* __restrict__ labels, MView M, int32_t* flags) {
@@ -805,12 +813,128 @@ __global__ void k_insert_edges(int64_t E, const int32_t* __restrict__ src,
     const int32_t a = labels[src[e]];
     const int32_t b = labels[dst[e]];
     const int32_t ww = w ? w[e] : 1;
+    const bool ia = !kSkipLds || !lds_row(M.cap[a], M.B);
+    const bool ib = !kSkipLds || !lds_row(M.cap[b], M.B);
     if (a == b) {
-      m_add<false>(M, a, a, ww, ww, flags);
+      if (ia) m_add<false>(M, a, a, ww, ww, flags);
     } else {
-      m_add<false>(M, a, b, ww, 0, flags);
-      m_add<false>(M, b, a, 0, ww, flags);
+      if (ia) m_add<false>(M, a, b, ww, 0, flags);
+      if (ib) m_add<false>(M, b, a, 0, ww, flags);
     }
+  }
+}
+
+// Nodes grouped by block (for the per-row build): node x goes to
+// nodes[off[labels[x]] + k], k its arrival order in its block.
+__global__ void k_node_lists(int32_t N, const int32_t* __restrict__ labels,
+                             const int32_t* __restrict__ off, int32_t* cur, int32_t* nodes) {
+  for (int32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < N; x += gridDim.x * blockDim.x) {
+    const int32_t a = labels[x];
+    nodes[off[a] + atomicAdd(&cur[a], 1)] = x;
+  }
+}
+
+// M build for large B: each hashed row of at most kRowTab slots by one
+// wavefront, from the lists of the row's own nodes alone -- an out-edge
+// x -> v adds to M[a][label v] (o), an in-edge u -> x to M[label u][a] (i) --
+// laid out in LDS and written out whole (a slab that size holds at most
+// (kRowTab - 16) / 2 list entries, so no wavefront has a long row).  The
+// per-edge build sends every edge to two rows' slabs at random instead; it
+// still builds the other rows (k_insert_edges<true>).
+__global__ __launch_bounds__(kWave* kNodalWaves) void k_build_rows(
+    GraphView g, const int32_t* __restrict__ labels, const int32_t* __restrict__ bl_off,
+    const int32_t* __restrict__ bl_nodes, MView M, int32_t* flags) {
+  __shared__ int32_t s_k[kNodalWaves][kRowTab];
+  __shared__ int32_t s_o[kNodalWaves][kRowTab];
+  __shared__ int32_t s_i[kNodalWaves][kRowTab];
+  __shared__ int32_t s_pre[kNodalWaves][kWave + 1];
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int32_t a = blockIdx.x * kNodalWaves + w;
+  if (a >= M.B) return;  // no workgroup barrier below
+  const int64_t o = M.off[a];
+  const int32_t c = M.cap[a];
+  if (!lds_row(c, M.B)) return;
+  int32_t* tk = s_k[w];
+  int32_t* to = s_o[w];
+  int32_t* ti = s_i[w];
+  int32_t* pre = s_pre[w];
+  for (int32_t j = lane; j < c; j += kWave) {
+    tk[j] = -1;
+    to[j] = 0;
+    ti[j] = 0;
+  }
+  const uint32_t mask = (uint32_t)c - 1;
+  const int32_t n0 = bl_off[a], n1 = a + 1 < M.B ? bl_off[a + 1] : g.N;
+  long nk = 0;  // keys this lane placed in the LDS slab
+  bool full = false;
+  for (int32_t base = n0; base < n1; base += kWave) {  // 64 nodes at a time
+    const int32_t j = base + lane;
+    int64_t o0 = 0, i0 = 0;
+    long ko = 0, kn = 0;
+    if (j < n1) {
+      const int32_t x = bl_nodes[j];
+      o0 = g.out_ptr[x];
+      i0 = g.in_ptr[x];
+      ko = g.out_ptr[x + 1] - o0;
+      kn = ko + (g.in_ptr[x + 1] - i0);
+    }
+    const long inc = wave_incl_scan(kn);
+    const long tot = __shfl(inc, kWave - 1);
+    wave_fence();  // the previous chunk's reads of pre are done
+    pre[lane + 1] = (int32_t)inc;
+    if (lane == 0) pre[0] = 0;
+    wave_fence();
+    for (long e0 = 0; e0 < tot; e0 += kWave) {
+      const int32_t ei = (int32_t)(e0 + lane);
+      int lo = 0, hi = kWave - 1;  // the node k of entry ei: pre[k] <= ei < pre[k + 1]
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= ei)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int64_t xo = __shfl(o0, lo), xi = __shfl(i0, lo);
+      const long xko = __shfl(ko, lo);
+      if (ei >= tot) continue;
+      const long r = ei - pre[lo];
+      const bool out = r < xko;
+      int32_t v, wt = 1;
+      if (out) {
+        v = g.out_idx[xo + r];
+        if (g.out_w) wt = g.out_w[xo + r];
+      } else {
+        v = g.in_idx[xi + (r - xko)];
+        if (g.in_w) wt = g.in_w[xi + (r - xko)];
+      }
+      const int32_t b = labels[v];
+      uint32_t h = hash32(b) & mask;
+      bool done = false;
+      for (int32_t probe = 0; probe < c && !done; ++probe) {
+        int32_t k = __hip_atomic_load(&tk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        if (k < 0) {
+          k = atomicCAS(&tk[h], -1, b);
+          if (k == -1) {
+            k = b;
+            ++nk;
+          }
+        }
+        if (k == b) {
+          atomicAdd(out ? &to[h] : &ti[h], wt);
+          done = true;
+        }
+        h = (h + 1) & mask;
+      }
+      full |= !done;
+    }
+  }
+  wave_fence();
+  for (int32_t j = lane; j < c; j += kWave) M.kv[o + j] = Slot{tk[j], 0, to[j], ti[j]};
+  nk = wave_sum(nk);
+  if (lane == 0) M.nkeys[a] = (int32_t)nk;
+  if (__ballot(full)) {
+    if (lane == 0) atomicOr(flags, kFlagFull);
   }
 }
 
@@ -843,6 +967,7 @@ __global__ __launch_bounds__(256) void k_insert_small(int64_t E, const int32_t* 
 // edges' block pairs a*B + b are sorted and summed per pair, then each pair
 // is added once.
 constexpr int32_t kPairSortB = 4096;
+constexpr int32_t kRowBuildB = 16384;  // from here up the build goes by rows (k_build_rows)
 
 __global__ void k_pair_keys(int64_t E, const int32_t* __restrict__ src,
                             const int32_t* __restrict__ dst, const int32_t* __restrict__ w,
@@ -909,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
 // The per-block counters and flags a rebuild starts from, zeroed in one
 // launch (degrees, edge counts, sizes, key counts, growth marks, parked
 // counts: B entries each; build_flags[0..2]).
-constexpr int kClearArrays = 8;
+constexpr int kClearArrays = 9;
 struct ClearArgs {
   int32_t* p[kClearArrays];
   int32_t n;
@@ -2617,6 +2742,8 @@ struct usap_engine {
   int32_t* labels = nullptr;
   int32_t *d_out = nullptr, *d_in = nullptr, *ecnt_out = nullptr, *ecnt_in = nullptr,
           *bsize = nullptr;
+  // nodes grouped by block for the per-row build: offsets, arrival cursors, nodes
+  int32_t *bl_off = nullptr, *bl_cur = nullptr, *bl_nodes = nullptr;
   DevSpM M;  // block rows (both orientations of M)
   int64_t* tmpM = nullptr;
   void* scan_tmp = nullptr;
@@ -2909,6 +3036,33 @@ int build_by_pairs(usap_engine* e) {
 }
 
 // Full M build from the labels (graph_parallel.hpp:703-759).
+// The M build for B >= kRowBuildB: nodes grouped by block (offsets by an
+// exclusive scan of the block sizes, then a scatter), k_build_rows for the
+// rows it lays out in LDS, per-edge inserts for the others.
+int build_rows(usap_engine* e) {
+  const int32_t B = e->B;
+  hipStream_t st = e->stream;
+  size_t bytes = 0;
+  HIPCHK(rocprim::exclusive_scan(nullptr, bytes, e->bsize, e->bl_off, (int32_t)0, (size_t)B,
+                                 rocprim::plus<int32_t>(), st));
+  if (bytes > e->scan_tmp_bytes) {
+    dfree(e, e->scan_tmp);
+    if (int rc = dalloc_raw(e, &e->scan_tmp, bytes)) return rc;
+    e->scan_tmp_bytes = bytes;
+  }
+  bytes = e->scan_tmp_bytes;
+  HIPCHK(rocprim::exclusive_scan(e->scan_tmp, bytes, e->bsize, e->bl_off, (int32_t)0, (size_t)B,
+                                 rocprim::plus<int32_t>(), st));
+  k_node_lists<<<std::min<unsigned>(blocks_for(e->N, 256), 4096u), 256, 0, st>>>(
+      e->N, e->labels, e->bl_off, e->bl_cur, e->bl_nodes);
+  k_build_rows<<<blocks_for(B, kNodalWaves), kWave * kNodalWaves, 0, st>>>(
+      gview(e), e->labels, e->bl_off, e->bl_nodes, view(e->M, B), e->build_flags);
+  k_insert_edges<true><<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
+      e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), e->build_flags);
+  HIPCHK(hipGetLastError());
+  return USAP_OK;
+}
+
 int rebuild_M(usap_engine* e) {
   const int32_t B = e->B;
   hipStream_t st = e->stream;
@@ -2916,7 +3070,7 @@ int rebuild_M(usap_engine* e) {
   {
     ClearArgs c;
     int32_t* arr[kClearArrays] = {e->d_out, e->d_in, e->ecnt_out, e->ecnt_in,
-                                  e->bsize, e->M.nkeys, e->M.gmark, e->M.gpend};
+                                  e->bsize, e->M.nkeys, e->M.gmark, e->M.gpend, e->bl_cur};
     for (int i = 0; i < kClearArrays; ++i) c.p[i] = arr[i];
     c.n = B;
     c.flags = e->build_flags;
@@ -2982,9 +3136,12 @@ int rebuild_M(usap_engine* e) {
                                           view(e->M, B));
     else if (B <= kPairSortB) {
       if (int rc = build_by_pairs(e)) return rc;
-    } else
-      k_insert_edges<<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
+    } else if (B < kRowBuildB) {
+      k_insert_edges<false><<<std::min<int64_t>(blocks_for(e->E, 256), 16384), 256, 0, st>>>(
           e->E, e->out_src, e->out_idx, e->out_w, e->labels, view(e->M, B), e->build_flags);
+    } else {
+      if (int rc = build_rows(e)) return rc;
+    }
   }
   k_bucket_sums<<<blocks_for(B, 4), 256, 0, st>>>(view(e->M, B));
   HIPCHK(hipGetLastError());
@@ -3114,6 +3271,9 @@ int usap_destroy(usap_engine* e) {
   dfree(e, e->ecnt_out);
   dfree(e, e->ecnt_in);
   dfree(e, e->bsize);
+  dfree(e, e->bl_off);
+  dfree(e, e->bl_cur);
+  dfree(e, e->bl_nodes);
   for (DevSpM* m : {&e->M}) {
     dfree(e, m->off);
     dfree(e, m->cap);
@@ -3519,6 +3679,9 @@ int alloc_state(usap_engine* e) {
   TRY(dalloc(e, &e->ecnt_out, N));
   TRY(dalloc(e, &e->ecnt_in, N));
   TRY(dalloc(e, &e->bsize, N));
+  TRY(dalloc(e, &e->bl_off, N));
+  TRY(dalloc(e, &e->bl_cur, N));
+  TRY(dalloc(e, &e->bl_nodes, N));
   for (DevSpM* m : {&e->M}) {
     TRY(dalloc(e, &m->off, (size_t)N + 1));
     TRY(dalloc(e, &m->cap, N));
