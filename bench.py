@@ -247,7 +247,7 @@ def measured_traffic(config):
     return val, src
 
 
-def batched_bench(n_graphs=256, workers=0, replay=0):
+def batched_bench(n_graphs=256, workers=0):
     """The batched workload (n_graphs seeded DCSBMs, N log-uniform in
     [500, 50000]) partitioned one graph at a time (the plain form: one
     engine, its launches back to back) and then with several graphs in
@@ -258,8 +258,7 @@ def batched_bench(n_graphs=256, workers=0, replay=0):
     graphs, specs = usap_amd.batch_graphs(n=n_graphs, seed=4)
     log(f"[bench] batched workload: {len(graphs)} graphs, {sum(g.N for g in graphs)} nodes, "
         f"{sum(g.E for g in graphs)} edges, generated in {time.perf_counter() - t:.1f}s")
-    # reference defaults (SCC init, ref dS); replay: sweeps from captured HIP graphs
-    params = [dict(seed=s["seed"], graphs=replay) for s in specs]
+    params = [dict(seed=s["seed"]) for s in specs]  # reference defaults (SCC init, ref dS)
     trials = []
     if workers == 0:
         t = time.perf_counter()
@@ -296,7 +295,7 @@ def batched_bench(n_graphs=256, workers=0, replay=0):
 
 
 def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interval_s=0.05,
-                 check=4, replay=0, final_partition=False, repartition_growth=2.0):
+                 check=4, final_partition=False, repartition_growth=2.0):
     """The served stream: n_graphs DCSBMs (N log-uniform in [1000, 200000])
     arriving by a Poisson process at each offered rate, each delivered as 10
     increments of 10% of its shuffled edges, interval_s apart; stage 1
@@ -309,7 +308,7 @@ def served_bench(n_graphs=24, rates=(0.5, 1.0, 2.0, 4.0, 8.0), workers=8, interv
     graphs = sv.served_graphs(n=n_graphs, seed=5)
     log(f"[bench] served workload: {len(graphs)} graphs, {sum(g.N for g in graphs)} nodes, "
         f"{sum(g.E for g in graphs)} edges, generated in {time.perf_counter() - t:.1f}s")
-    params = [dict(seed=g.seed, graphs=replay) for g in graphs]
+    params = [dict(seed=g.seed) for g in graphs]
     points, mixes = [], []
     for rate in rates:
         arr = sv.poisson_arrivals(len(graphs), rate, seed=5)
@@ -411,8 +410,6 @@ def main():
                     help="skip the SCC-init partition and the batched workload")
     ap.add_argument("--graphs", type=int, default=256, help="batched workload size (c4)")
     ap.add_argument("--workers", type=int, default=0, help="graphs in flight (c4; 0 = autotune)")
-    ap.add_argument("--replay", action="store_true",
-                    help="c4/c5: replay nodal sweeps from captured HIP graphs")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -426,14 +423,14 @@ def main():
         return
     if args.config == "c5":
         out = served_bench(n_graphs=args.graphs if args.graphs != 256 else 24,
-                           workers=args.workers or 8, replay=int(args.replay))
+                           workers=args.workers or 8)
         out.update({"unit": "graphs/s", "n_gpus": 1, "higher_is_better": True,
                     "scaling": "weak", "vs_baseline": None, "dtype": "int128",
                     "data": "synthetic (seeded directed DCSBM generator)"})
         print(json.dumps(out), flush=True)
         return
     if args.config == "c4":
-        out = batched_bench(args.graphs, args.workers, replay=int(args.replay))
+        out = batched_bench(args.graphs, args.workers)
         out.update({"unit": "graphs/s", "n_gpus": 1, "higher_is_better": True,
                     "scaling": "weak", "vs_baseline": None, "dtype": "int128",
                     "data": "synthetic (seeded directed DCSBM generator)"})

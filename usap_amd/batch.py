@@ -39,10 +39,11 @@ def _arrays(graphs, params):
     return gs, ps
 
 
-def partition_batch(graphs, params, workers=0):
+def partition_batch(graphs, params, workers=8):
     """Partitions every graph (params: one dict for all, or one per graph),
-    `workers` graphs in flight (0 = autotuned).  Returns (labels list,
-    stats list, batch stats dict)."""
+    `workers` graphs in flight (8, the autotuner's usual pick on one MI355X,
+    by default; 0 = autotuned).  Returns (labels list, stats list, batch
+    stats dict)."""
     n = len(graphs)
     gs, ps = _arrays(graphs, params)
     off = np.zeros(n + 1, np.int64)
