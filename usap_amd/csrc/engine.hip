@@ -147,21 +147,19 @@ struct MView {
   int32_t* nkeys;
   Slot* kv;        // row slabs
   int32_t* bs;     // [B][kNBK]: sum of the row's o + i per key bucket
-  int32_t* gmark;  // [B]: row queued for growth
-  int32_t* gq;     // [B]: growth queue
   int32_t* gpend;  // [B]: increments of the row waiting in the overflow buffer
   int32_t* ovf;    // [4][ovf_cap]: (row, key, do, di) increments that found no free slot
   int32_t ovf_cap;
   struct GrowState* gst;
   int32_t B;
   int32_t bshift;  // bucket of key k = k >> bshift
-  int32_t slack;   // slab_slack: a hashed row is queued for growth past 1/slack load
+  int32_t slack;   // slab_slack: a hashed row grows past 1/slack load
 };
 
 // Rows to grow between two batches; slabs for grown rows come from the free
 // tail of the pool.
 struct GrowState {
-  int32_t n;                  // rows queued by the last k_apply
+  int32_t n;                  // unused (rows to grow are found by k_grow's scan)
   int32_t nospace;            // the pool ran out: the host rebuilds M
   int32_t ovf_n;              // entries in the overflow buffer
   unsigned long long used;    // pool slots in use
@@ -280,16 +278,16 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
       }
       k = atomicCAS(&m.kv[o + h].k, -1, b);
       if (k == -1) {
-        const int32_t n = atomicAdd(&m.nkeys[a], 1) + 1;
-        // past 1/slack load during nodal batches (short probe chains for
-        // the lookups, most of which miss) or 3/4 in a build: grow / rebuild
-        if (kBuckets ? m.slack * n > c : 4 * n > 3 * c) {
-          if (!kBuckets) {
-            atomicOr(flags, kFlagLoad);
-          } else if (atomicCAS(&m.gmark[a], 0, 1) == 0) {  // queue the row once
-            m.gq[atomicAdd(&m.gst->n, 1)] = a;
-          }
+        if (kBuckets) {
+          // the counts and the key count without waiting for either: k_grow
+          // finds the rows past 1/slack load (short probe chains for the
+          // lookups, most of which miss) from the key counts themselves
+          cnt_add(&m.kv[o + h], dout, din);
+          atomicAdd(&m.nkeys[a], 1);
+          return;
         }
+        // past 3/4 load in a build: rebuild
+        if (4 * (atomicAdd(&m.nkeys[a], 1) + 1) > 3 * c) atomicOr(flags, kFlagLoad);
         k = b;
       }
     }
@@ -307,8 +305,7 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
       m.ovf[m.ovf_cap + i] = b;
       m.ovf[2 * m.ovf_cap + i] = dout;
       m.ovf[3 * m.ovf_cap + i] = din;
-      atomicAdd(&m.gpend[a], 1);
-      if (atomicCAS(&m.gmark[a], 0, 1) == 0) m.gq[atomicAdd(&m.gst->n, 1)] = a;
+      atomicAdd(&m.gpend[a], 1);  // k_grow grows every row with parked increments
       return;
     }
   }
@@ -1034,7 +1031,7 @@ __global__ __launch_bounds__(256) void k_bucket_sums(MView m) {
 // The per-block counters and flags a rebuild starts from, zeroed in one
 // launch (degrees, edge counts, sizes, key counts, growth marks, parked
 // counts: B entries each; build_flags[0..2]).
-constexpr int kClearArrays = 9;
+constexpr int kClearArrays = 8;
 struct ClearArgs {
   int32_t* p[kClearArrays];
   int32_t n;
@@ -1909,8 +1906,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_nodal(NodalArgs a) {
   const int w = threadIdx.x / kWave;
   if (a.tstamp && threadIdx.x == 0) a.tstamp[blockIdx.x] = wall_clock64();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.M.gst->n = 0;  // the previous k_grow consumed the queues
-    a.M.gst->ovf_n = 0;
+    a.M.gst->ovf_n = 0;  // the previous k_grow replayed the parked increments
     // k_apply decides by this copy whether the batch runs: its block 0
     // updates stop meanwhile
     a.sweep->stop_n = a.sweep->stop;
@@ -2426,8 +2422,9 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
   }
 }
 
-// Rows queued by k_apply (key count past 1/slack of the slab, or an increment
-// parked), handled between two batches, dead keys (both counts 0) dropped: a
+// Rows whose key count passed 1/slack of the slab or that have parked
+// increments (each of the 512 workgroups scans its slice of the rows),
+// handled between two batches, dead keys (both counts 0) dropped: a
 // row whose live keys and parked increments fit 1/(2 slack) of its slab (and
 // at most kCompact slots) is compacted in place through LDS; any other row
 // moves to a slab >= 2*slack*live+16 (direct once that reaches B/8) at the
@@ -2438,6 +2435,7 @@ __global__ __launch_bounds__(kWave* kNodalWaves) void k_apply(ApplyArgs a) {
 constexpr int kCompact = 4096;
 constexpr int kGrowLds = 2048;    // slabs up to this many slots are laid out in LDS
 constexpr int kGrowBlocks = 512;  // k_grow's grid
+constexpr int kGrowScan = 1024;   // rows a k_grow workgroup checks per round (4 per thread)
 constexpr int32_t kOvfCap = 1 << 22;  // parked increments
 
 struct GrowSide {
@@ -2445,8 +2443,6 @@ struct GrowSide {
   int32_t* cap;
   int32_t* nkeys;
   Slot* kv;
-  int32_t* gmark;
-  const int32_t* gq;
   int32_t* gpend;
   const int32_t* ovf;
   int32_t ovf_cap;
@@ -2536,8 +2532,9 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
   __shared__ int32_t t_k[kGrowLds];  // its new slab, when it fits
   __shared__ int32_t t_o[kGrowLds];
   __shared__ int32_t t_i[kGrowLds];
+  __shared__ int32_t s_q[kGrowScan];  // rows of the scanned chunk to handle
   __shared__ long long s_base;
-  __shared__ int32_t s_cap, s_live;
+  __shared__ int32_t s_cap, s_live, s_nq;
   const int32_t B = ga.B;
   const GrowSide& g = ga.side;
   if (ga.tstamp && threadIdx.x == 0) ga.tstamp[blockIdx.x] = wall_clock64();
@@ -2555,9 +2552,35 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
       ga.status->flags = 0;
     }
   }
-  const int32_t nq = g.gst->n;
-  for (int32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
-    const int32_t a = g.gq[qi];
+  // the rows to grow or compact, found by a scan of this workgroup's slice
+  // of the rows: hashed rows past 1/slack load, or with parked increments
+  // (k_apply only counts keys and parked increments; it queues nothing)
+  const int32_t per = (B + (int32_t)gridDim.x - 1) / (int32_t)gridDim.x;
+  const int32_t lo = (int32_t)blockIdx.x * per, hi = min(B, lo + per);
+  for (int32_t base = lo; base < hi; base += kGrowScan) {
+    if (threadIdx.x == 0) s_nq = 0;
+    __syncthreads();
+    int32_t cq[kGrowScan / 256], nq_k[kGrowScan / 256], pq[kGrowScan / 256];
+#pragma unroll
+    for (int q = 0; q < kGrowScan / 256; ++q) {
+      const int32_t r = base + q * 256 + (int32_t)threadIdx.x;
+      cq[q] = B;
+      nq_k[q] = 0;
+      pq[q] = 0;
+      if (r < hi) {
+        cq[q] = g.cap[r];
+        nq_k[q] = g.nkeys[r];
+        pq[q] = g.gpend[r];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kGrowScan / 256; ++q)
+      if (cq[q] < B && ((int64_t)g.slack * nq_k[q] > cq[q] || pq[q] > 0))
+        s_q[atomicAdd(&s_nq, 1)] = base + q * 256 + (int32_t)threadIdx.x;
+    __syncthreads();
+    const int32_t nq = s_nq;
+    for (int32_t qi = 0; qi < nq; ++qi) {
+    const int32_t a = s_q[qi];
     const int64_t o = g.off[a];
     const int32_t c = g.cap[a];
     const int32_t pend = g.gpend[a];  // parked increments (at most one new key each)
@@ -2679,11 +2702,9 @@ __global__ __launch_bounds__(256) void k_grow(GrowArgs ga) {
     }
     if (pend > 0) replay_row(g, a, o2, c2, B, ga.sweep);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      g.gmark[a] = 0;
-      g.gpend[a] = 0;
-    }
+    if (threadIdx.x == 0) g.gpend[a] = 0;
     __syncthreads();
+    }
   }
   if (ga.tstamp) {
     __syncthreads();
@@ -2704,8 +2725,6 @@ struct DevSpM {
   Slot* kv = nullptr;     // row slabs [P]
   int32_t* bs = nullptr;  // [bs_rows][kNBK]
   int32_t bs_rows = 0;
-  int32_t* gmark = nullptr;  // [N]
-  int32_t* gq = nullptr;     // [N]
   int32_t* gpend = nullptr;  // [N]
   int32_t* ovf = nullptr;    // [4][ovf_cap]
   int32_t ovf_cap = 0;       // parked increments that fit
@@ -2937,8 +2956,6 @@ MView view(const DevSpM& m, int32_t B) {
   v.nkeys = m.nkeys;
   v.kv = m.kv;
   v.bs = m.bs;
-  v.gmark = m.gmark;
-  v.gq = m.gq;
   v.gpend = m.gpend;
   v.ovf = m.ovf;
   v.ovf_cap = m.ovf_cap;
@@ -3070,7 +3087,7 @@ int rebuild_M(usap_engine* e) {
   {
     ClearArgs c;
     int32_t* arr[kClearArrays] = {e->d_out, e->d_in, e->ecnt_out, e->ecnt_in,
-                                  e->bsize, e->M.nkeys, e->M.gmark, e->M.gpend, e->bl_cur};
+                                  e->bsize, e->M.nkeys, e->M.gpend, e->bl_cur};
     for (int i = 0; i < kClearArrays; ++i) c.p[i] = arr[i];
     c.n = B;
     c.flags = e->build_flags;
@@ -3280,8 +3297,6 @@ int usap_destroy(usap_engine* e) {
     dfree(e, m->nkeys);
     dfree(e, m->kv);
     dfree(e, m->bs);
-    dfree(e, m->gmark);
-    dfree(e, m->gq);
     dfree(e, m->gpend);
     dfree(e, m->ovf);
     dfree(e, m->gst);
@@ -3686,8 +3701,6 @@ int alloc_state(usap_engine* e) {
     TRY(dalloc(e, &m->off, (size_t)N + 1));
     TRY(dalloc(e, &m->cap, N));
     TRY(dalloc(e, &m->nkeys, N));
-    TRY(dalloc(e, &m->gmark, N));
-    TRY(dalloc(e, &m->gq, N));
     TRY(dalloc(e, &m->gpend, N));
     m->ovf_cap = (int32_t)std::min<int64_t>(kOvfCap, 2 * e->E + 4096);
     TRY(dalloc(e, &m->ovf, (size_t)4 * m->ovf_cap));
@@ -4388,8 +4401,6 @@ int launch_batch(usap_engine* e, int32_t beg, int32_t end, int32_t bidx, size_t 
     gs.cap = m.cap;
     gs.nkeys = m.nkeys;
     gs.kv = m.kv;
-    gs.gmark = m.gmark;
-    gs.gq = m.gq;
     gs.gpend = m.gpend;
     gs.ovf = m.ovf;
     gs.ovf_cap = m.ovf_cap;
