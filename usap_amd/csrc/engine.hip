@@ -16,10 +16,12 @@
 //                   and one 64-bit atomic updates both counts.
 // Keys are never deleted inside a sweep (counts may drop to 0).  Each row
 // also keeps kNBK bucket sums (sum of o + i per key range) for the two-level
-// inverse-CDF pick over the combined weights M[a][b] + M[b][a].  A row whose key count passes 1/4 of its slab
-// during nodal batches is queued; an increment that finds no free slot is parked in an overflow
-// buffer; between two batches k_grow compacts queued rows in place or moves
-// them to a larger slab at the pool's tail and replays the parked increments.
+// inverse-CDF pick over the combined weights M[a][b] + M[b][a].  An
+// increment that finds no free slot near its key during nodal batches is
+// parked in an overflow buffer; between two batches k_grow finds the rows
+// whose key count passed 1/slack of the slab or that have parked increments,
+// compacts them in place or moves them to a larger slab at the pool's tail,
+// and replays the parked increments.
 // The host rebuilds M from the labels only when the pool runs out.
 //
 // Kernels (one wavefront of 64 lanes per proposal item):
@@ -250,8 +252,9 @@ constexpr int32_t kMaxWalk = 16;  // probes before a nodal-batch increment is pa
 // that exists), given the key k read at the first probed slot `slot`.  Keys
 // only go from empty to set, so an earlier read is a valid first read: a
 // stale "empty" makes the claim fail and return the key that is there.
-// kBuckets also updates the row's bucket sum and queues rows for growth
-// instead of flagging a rebuild.
+// kBuckets also updates the row's bucket sum, and leaves a crowded row to
+// k_grow (by its key count, or a parked increment) instead of flagging a
+// rebuild.
 template <bool kBuckets>
 __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32_t b, int32_t dout,
                                          int32_t din, int64_t slot, int32_t k, int32_t* flags) {
@@ -268,7 +271,7 @@ __device__ __forceinline__ void m_add_at(const MView& m, int32_t a, Row R, int32
   uint32_t h = (uint32_t)(slot - o);
   for (int32_t probe = 0; probe < c; ++probe) {
     // an increment whose key is still absent after kMaxWalk slots is parked
-    // (the row is queued for growth) rather than walking a crowded row
+    // (k_grow grows the row) rather than walking a crowded row
     if (kBuckets && !dec && probe >= kMaxWalk) break;
     if (probe > 0) k = ld_relaxed(&m.kv[o + h].k);
     if (k < 0) {
@@ -1083,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_stamp_reduce(const unsigned long long* 
   }
 }
 
-// Growth state after a rebuild: nothing queued or parked, the pool used up to
+// Growth state after a rebuild: nothing parked, the pool used up to
 // the new slabs' end (the event counters are kept).
 __global__ void k_reset_grow(GrowState* g, unsigned long long used) {
   g->n = 0;
@@ -3127,7 +3130,7 @@ int rebuild_M(usap_engine* e) {
   m.total = tot;
   m.slack = e->p.slab_slack;
   // every row direct-mapped (cap B each; a hashed row's cap is below B/8):
-  // no insert probes, so no row is ever queued or an increment parked until
+  // no insert probes, so no row ever needs growth or an increment parked until
   // the next rebuild, and k_grow only has the batch's error flags to fold
   e->all_direct = tot == (int64_t)B * B;
   if (3 * m.total > m.P) {  // two thirds of the pool is headroom for rows that grow
