@@ -282,20 +282,21 @@ def test_c3_full_size_state_invariants(usap):
             _check_state(usap, eng, g, lam)
 
 
-def test_row_build_large_B_weighted(usap):
-    """The per-row M build (B >= 16384) on a weighted graph with self-loops,
-    hubs and isolated nodes, with blocks of one node to thousands: small rows
-    laid out in LDS, hashed rows above the LDS slab and a direct row inserted
-    in place.  M, the degrees and the MDL equal the host recompute after the
-    build and after a nodal sweep that moves nodes between such rows."""
+@pytest.mark.parametrize("B", [20000, 10000, 2000])
+def test_row_build_large_B_weighted(usap, B):
+    """The M builds on a weighted graph with self-loops, hubs and isolated
+    nodes, with blocks of one node to thousands: per row (B >= 16384: small
+    rows laid out in LDS, hashed rows above the LDS slab and a direct row
+    inserted in place), edge by edge (B = 10000) and by sorted block pairs
+    (B = 2000).  M, the degrees and the MDL equal the host recompute after
+    the build and after a nodal sweep that moves nodes between such rows."""
     g = random_graph(40000, 400000, seed=21, weighted=True, self_loops=True, hubs=3,
                      isolated=50)
     lam = usap.lambda_table(int(g.out_w.sum()))
-    B = 20000
     lab = np.empty(g.N, np.int32)
     lab[:3000] = 0                                # a direct row (the hubs are here)
-    lab[3000:3800] = 1 + np.arange(800) % 20      # hashed rows of 2048 slots
-    lab[3800:] = np.arange(g.N - 3800) % B        # one or two nodes: rows in LDS
+    lab[3000:3800] = 1 + np.arange(800) % 20      # 20 blocks of 40 nodes
+    lab[3800:] = np.arange(g.N - 3800) % B        # every block used, most small
     eng = usap.Engine(g, seed=5, nodal_batch=4096)
     eng.set_labels(lab, B)
     _flush(f"row build B={eng.B}")
